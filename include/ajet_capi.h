@@ -1,0 +1,141 @@
+/*
+ * ajet_capi.h -- C ABI of the MI355X (gfx950) RK4 time step of the axisymmetric jet solver.
+ *
+ * This is the drop-in boundary for the reference's hot path. The reference drives that path
+ * from PROGRAM ModSolver with two argument-less module procedures over global module state:
+ *     DO iteration = ...; CALL ComputeTimeStep; CALL RK4; ENDDO     (ModSolver.f90:122-125)
+ * The Fortran host of this project keeps those entry points (ComputeTimeStep, RK4) and
+ * implements their bodies with the calls below through ISO_C_BINDING.  Nothing here uses
+ * torch types: plain pointers, sizes and POD structs only.
+ *
+ * Conventions
+ *   - Host arrays are Fortran column-major.  A "field" is (nz, nr) with z (index i) fastest;
+ *     a "state" is (nz, nr, 5) holding the primitives rho, Vr, Vtheta, Vz, T in that order
+ *     (the pv(:,:,1,1:5) array of ModuleVariables.f90:39).
+ *   - Return value: 0 on success, a negative AJET_E* code for a library error, a positive
+ *     value for a hipError_t.  ajet_last_error() gives the message.
+ *   - One context owns its device buffers and one HIP stream.  Calls on one context are not
+ *     thread-safe.  Work is stream ordered; only ajet_get_* and ajet_compute_dt synchronise.
+ */
+#ifndef AJET_CAPI_H
+#define AJET_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  AJET_OK = 0,
+  AJET_EINVAL = -1,        /* bad argument (null pointer, size, flag) */
+  AJET_EGRID = -2,         /* grid is not a tensor-product (z=z(i), r=r(j)) grid */
+  AJET_EPATCH = -3,        /* a bc line gives a patch outside the grid or an unusable layout */
+  AJET_ENODEV = -4,        /* no usable gfx950 device */
+  AJET_EBLOWUP = -5,       /* time step blow-up: every point blanked (ModInputOutput.f90:424-428) */
+  AJET_ENONFINITE = -6,    /* non-finite value met in the time-step reduction */
+  AJET_ETILE = -7,         /* a boundary closure does not fit the kernel tiling */
+  AJET_EUNSUPPORTED = -8   /* feature not available in this build */
+};
+
+/* Boundary-condition codes of the bc file (ModuleVariables.f90:112-117). */
+enum {
+  AJET_BC_AXIS = 90,
+  AJET_BC_SAT_FARFIELD = 91,
+  AJET_BC_NSCBC_NONREFLECTING = 92,
+  AJET_BC_SAT_NOSLIP_WALL = 93,
+  AJET_BC_NSCBC_NOSLIP_WALL = 94,
+  AJET_BC_SPONGE = 95
+};
+
+/* Solver constants.  The reference keeps these as module variables and parameters
+ * (ModuleVariables.f90:12-13, 66, 71, 73, 152, 164, 177); ajet_params_default() fills the
+ * reference's values. */
+typedef struct ajet_params {
+  int32_t nz, nr;            /* grid points along z (i) and r (j) */
+  int32_t viscous;           /* do_viscous_terms */
+  int32_t shock_capturing;   /* ShockCapturing: localized artificial diffusivity */
+  int32_t forcing;           /* inflow excitation (0 = off, the reference default) */
+  int32_t reserved;
+  double gam, Re, Pr, CFL;
+  double C_mu, C_beta, C_kappa;
+  double sponge_start_top, sponge_A_top, sponge_end_left, sponge_A_left;
+} ajet_params;
+
+/* One line of the bc file as ModPatch.f90:49 reads it: grid, bc, bc_dir, is, ie, js, je
+ * (1-based; -1 stands for the last grid index).  The trailing ks, ke columns are unused. */
+typedef struct ajet_bc {
+  int64_t grid, bc, bc_dir, i_start, i_end, j_start, j_end;
+} ajet_bc;
+
+/* A 4-deep closure patch as ModPatch.f90:264-309 builds it (patch_type,
+ * ModuleVariables.f90:102-106), 1-based inclusive. */
+typedef struct ajet_patch {
+  int64_t i_start, i_end, j_start, j_end, bc, bc_dir;
+} ajet_patch;
+
+typedef struct ajet_ctx ajet_ctx;
+
+/* Fill *p with the reference constants for an nz x nr grid. */
+void ajet_params_default(ajet_params* p, int32_t nz, int32_t nr);
+
+/* Build the closure patches of a bc file for a single-domain nz x nr grid
+ * (CreateAllPatches/PatchCreation, ModPatch.f90:117-309).  Writes at most max_out patches and
+ * returns the number built (>= 0) or a negative error code. */
+int ajet_build_patches(int32_t nz, int32_t nr, const ajet_bc* lines, int32_t nlines,
+                       ajet_patch* out, int32_t max_out);
+
+/* Create a context: grid coordinates z, r and IBLANK are (nz, nr) fields
+ * (AssignGridCoordinates, ModInputOutput.f90:69-89); the bc lines replace the bc file.
+ * Computes the metrics (ModSolver.f90:57-76, ComputeMetric ModDeriv.f90:241-382) and the
+ * per-point patch maps (ModPatch.f90:214-242).  iblank may be NULL (all points active). */
+int ajet_create(const ajet_params* p, const double* z, const double* r, const int32_t* iblank,
+                const ajet_bc* lines, int32_t nlines, ajet_ctx** out);
+
+int ajet_destroy(ajet_ctx* ctx);
+const char* ajet_last_error(const ajet_ctx* ctx);
+
+/* Load the state and the target (SolnRead, ModInputOutput.f90:91-148), both (nz, nr, 5).
+ * Resets the stale time-step temperature to 0 as a fresh start or restart does. */
+int ajet_set_state(ajet_ctx* ctx, const double* pv, const double* pv_target);
+/* Copy the state back (the pv array WriteOutput gathers, ModInputOutput.f90:201-263). */
+int ajet_get_state(ajet_ctx* ctx, double* pv);
+
+/* ComputeTimeStep (ModInputOutput.f90:389-438): the CFL time step of the current state,
+ * kept on the device for the next ajet_rk4_step and copied to *dt (may be NULL to skip the
+ * copy and the synchronisation). */
+int ajet_compute_dt(ajet_ctx* ctx, double* dt);
+
+/* RK4 (ModRungeKutta.f90:5-171) without its output: advances the state by the device-side
+ * time step of the last ajet_compute_dt (or of the previous step's fused reduction). */
+int ajet_rk4_step(ajet_ctx* ctx);
+
+/* nsteps iterations of {ComputeTimeStep; RK4} (ModSolver.f90:122-125).  The time step of
+ * each iteration is reduced on the device inside the previous step's last stage; the
+ * sequence is appended to the device time-step log (see ajet_get_dt_log). */
+int ajet_run(ajet_ctx* ctx, int32_t nsteps);
+
+/* Time steps and accumulated time since ajet_set_state (ModRungeKutta.f90:152-155).
+ * Copies min(n, logged) entries to dts (may be NULL) and returns the number logged. */
+int ajet_get_dt_log(ajet_ctx* ctx, double* dts, int32_t n, double* time);
+
+/* Test hook: one ComputeRHS (ModRHS.f90:5-258) of the state q (nz, nr, 5), with AssignValues
+ * (ModAssign.f90:5-29) before it; writes rhs (nz, nr, 5).  Does not change the context state. */
+int ajet_rhs(ajet_ctx* ctx, const double* q, double* rhs);
+
+/* Test hook: the artificial transport coefficients mu*, beta*, kappa* that
+ * ComputeArtTransCoeffs (ModShockCapturing.f90:5-106) adds for the state q; each (nz, nr). */
+int ajet_lad_coeffs(ajet_ctx* ctx, const double* q, double* mustar, double* betastar,
+                    double* kappastar);
+
+/* The HIP stream the context launches on (a hipStream_t), for callers that time it. */
+void* ajet_stream(ajet_ctx* ctx);
+
+/* Library version string. */
+const char* ajet_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AJET_CAPI_H */

@@ -1,0 +1,85 @@
+"""Write the golden fixtures from the reference solver itself.
+
+Runs oracle/_ref/ref_harness -- the reference's own Fortran modules, compiled unmodified from
+/root/reference by oracle/build_ref.sh, driven in PROGRAM ModSolver's order -- on synthetic
+cases from axisymmetricjet_amd.jetcase, and stores its outputs as .npz fixtures next to this
+script.  Only this container can run it (the reference is not present elsewhere); the fixtures
+are data: inputs are regenerated from their seeds, and a SHA-256 of each input is stored so a
+changed generator is caught.
+
+    python tests/golden/make_golden.py [case ...]
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+from axisymmetricjet_amd import jetcase as jc  # noqa: E402
+
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+from cases import LAD_C, cases, input_digest  # noqa: E402
+
+
+def run_harness(tmp, paths, case, mode, shock, nsteps=1, out="o_"):
+    nml = os.path.join(tmp, f"{mode}.nml")
+    with open(nml, "w") as f:
+        f.write("&harness\n")
+        f.write(f" h_grid='{paths['grid']}', h_ic='{paths['ic']}', h_target='{paths['target']}',\n")
+        f.write(f" h_bc='{paths['bc']}', h_re={case.re!r}d0, h_pr={case.pr!r}d0, h_cfl=0.5d0,\n")
+        f.write(f" h_shock={shock}, h_mode='{mode}', h_nsteps={nsteps}, h_out='{os.path.join(tmp, out)}',\n")
+        f.write(f" h_cmu={LAD_C[0]!r}d0, h_cbeta={LAD_C[1]!r}d0, h_ckappa={LAD_C[2]!r}d0\n/\n")
+    with open(os.path.join(tmp, f"{mode}.log"), "w") as log:
+        subprocess.check_call([HARNESS, nml], cwd=tmp, stdout=log, stderr=subprocess.STDOUT)
+
+
+def make(name):
+    factory, shock, nsteps, extra = cases()[name]
+    case = factory()
+    nz, nr = case.nz, case.nr
+    out = {"input_sha256": np.array(input_digest(case)), "shock": np.array(shock),
+           "nsteps": np.array(nsteps), "lad_c": np.array(LAD_C)}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        paths = jc.write_case(os.path.join(tmp, "c_"), case)
+        run_harness(tmp, paths, case, "rhs", shock)
+        rd = lambda fn: np.fromfile(os.path.join(tmp, fn), "<f8")  # noqa: E731
+        out["rhs"] = rd("o_rhs.bin").reshape((nz, nr, 5), order="F")
+        out["tables"] = rd("o_tables.bin")
+        out["metrics"] = rd("o_metrics.bin").reshape((nz, nr, 4), order="F")
+        if shock:
+            lad = rd("o_lad.bin").reshape((nz, nr, 14), order="F")
+            out["lad_zfiltered"] = lad[:, :, 0:3]
+            out["lad_rfiltered"] = lad[:, :, 3:6]
+            out["lad_d4"] = lad[:, :, 6:12]
+            out["lad_S_dil"] = lad[:, :, 12:14]
+        if "ladrhs" in extra:
+            run_harness(tmp, paths, case, "ladrhs", shock)
+            out["ladrhs"] = rd("o_ladrhs.bin").reshape((nz, nr, 5), order="F")
+        run_harness(tmp, paths, case, "steps", shock, nsteps=nsteps)
+        raw = rd("o_dt.bin")
+        out["dts"] = raw[:nsteps]
+        out["time"] = raw[nsteps]
+        out["state"] = rd("o_state.bin").reshape((nz, nr, 5), order="F")
+    np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+    print(f"{name}: wrote {name}.npz  dt[0..2]={out['dts'][:3]}  |state|max={np.abs(out['state']).max():.6g}")
+
+
+def main(argv):
+    if not os.path.exists(HARNESS):
+        subprocess.check_call(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")])
+    names = argv or list(cases())
+    for n in names:
+        make(n)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
