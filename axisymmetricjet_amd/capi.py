@@ -1,0 +1,193 @@
+"""ctypes binding of libajet.so (include/ajet_capi.h) and a small host-side mirror of the
+reference's driver: ModSolver's setup, then {ComputeTimeStep; RK4} per iteration
+(ModSolver.f90:19-125).  Loading fails loudly if the HIP library is not built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libajet.so")
+
+AJET_OK = 0
+ERRORS = {-1: "EINVAL", -2: "EGRID", -3: "EPATCH", -4: "ENODEV", -5: "EBLOWUP", -6: "ENONFINITE",
+          -7: "ETILE", -8: "EUNSUPPORTED"}
+
+EXPORTS = ["ajet_params_default", "ajet_build_patches", "ajet_create", "ajet_destroy", "ajet_last_error",
+           "ajet_set_state", "ajet_get_state", "ajet_compute_dt", "ajet_rk4_stage", "ajet_rk4_step", "ajet_run",
+           "ajet_get_dt_log", "ajet_rhs", "ajet_lad_coeffs", "ajet_stream", "ajet_version"]
+
+
+class Params(C.Structure):
+    _fields_ = [("nz", C.c_int32), ("nr", C.c_int32), ("viscous", C.c_int32),
+                ("shock_capturing", C.c_int32), ("forcing", C.c_int32), ("reserved", C.c_int32),
+                ("gam", C.c_double), ("Re", C.c_double), ("Pr", C.c_double), ("CFL", C.c_double),
+                ("C_mu", C.c_double), ("C_beta", C.c_double), ("C_kappa", C.c_double),
+                ("sponge_start_top", C.c_double), ("sponge_A_top", C.c_double),
+                ("sponge_end_left", C.c_double), ("sponge_A_left", C.c_double)]
+
+
+class BcLine(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("grid", "bc", "bc_dir", "i_start", "i_end", "j_start", "j_end")]
+
+
+class Patch(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("i_start", "i_end", "j_start", "j_end", "bc", "bc_dir")]
+
+
+class AjetError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} is not built: run `python -m axisymmetricjet_amd.build`")
+        L = C.CDLL(LIB)
+        dp = np.ctypeslib.ndpointer(np.float64, flags="F_CONTIGUOUS")
+        vp = C.c_void_p
+        L.ajet_params_default.argtypes = [C.POINTER(Params), C.c_int32, C.c_int32]
+        L.ajet_params_default.restype = None
+        L.ajet_build_patches.argtypes = [C.c_int32, C.c_int32, C.POINTER(BcLine), C.c_int32, C.POINTER(Patch), C.c_int32]
+        L.ajet_create.argtypes = [C.POINTER(Params), dp, dp, vp, C.POINTER(BcLine), C.c_int32, C.POINTER(vp)]
+        L.ajet_destroy.argtypes = [vp]
+        L.ajet_last_error.argtypes = [vp]
+        L.ajet_last_error.restype = C.c_char_p
+        L.ajet_set_state.argtypes = [vp, dp, vp]
+        L.ajet_get_state.argtypes = [vp, dp]
+        L.ajet_compute_dt.argtypes = [vp, vp]
+        L.ajet_rk4_step.argtypes = [vp]
+        L.ajet_rk4_stage.argtypes = [vp, C.c_int32]
+        L.ajet_run.argtypes = [vp, C.c_int32]
+        L.ajet_get_dt_log.argtypes = [vp, vp, C.c_int32, vp]
+        L.ajet_rhs.argtypes = [vp, dp, dp]
+        L.ajet_lad_coeffs.argtypes = [vp, dp, dp, dp, dp]
+        L.ajet_stream.argtypes = [vp]
+        L.ajet_stream.restype = vp
+        L.ajet_version.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def default_params(nz, nr, **kw) -> Params:
+    p = Params()
+    lib().ajet_params_default(C.byref(p), nz, nr)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def bc_array(lines):
+    arr = (BcLine * max(1, len(lines)))()
+    for k, l in enumerate(lines):
+        arr[k] = BcLine(*[int(x) for x in l[:7]])
+    return arr, len(lines)
+
+
+def build_patches(nz, nr, lines):
+    arr, n = bc_array(lines)
+    out = (Patch * 64)()
+    m = lib().ajet_build_patches(nz, nr, arr, n, out, 64)
+    if m < 0:
+        raise AjetError(m, "ajet_build_patches")
+    return [(p.i_start, p.i_end, p.j_start, p.j_end, p.bc, p.bc_dir) for p in out[:m]]
+
+
+def F(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
+class Context:
+    """One device context: the time-step state of one jet run on the current GPU."""
+
+    def __init__(self, case=None, *, params: Params | None = None, z=None, r=None, iblank=None, bc_lines=None,
+                 **pkw):
+        L = lib()
+        if case is not None:
+            z, r, iblank, bc_lines = case.z, case.r, case.iblank, case.bc_lines
+            pkw.setdefault("Re", case.re)
+            pkw.setdefault("Pr", case.pr)
+        z, r = F(z), F(r)
+        nz, nr = z.shape
+        self.nz, self.nr = nz, nr
+        self.params = params or default_params(nz, nr, **pkw)
+        self._ib = None if iblank is None else np.asfortranarray(iblank, dtype=np.int32)
+        bcs, n = bc_array(bc_lines or [])
+        h = C.c_void_p()
+        rc = L.ajet_create(C.byref(self.params), z, r, None if self._ib is None else self._ib.ctypes.data,
+                           bcs, n, C.byref(h))
+        self.h = h
+        if rc != AJET_OK:
+            msg = L.ajet_last_error(h).decode() if h.value else ""
+            if h.value:
+                L.ajet_destroy(h)
+            self.h = None
+            raise AjetError(rc, msg)
+        if case is not None:
+            self.set_state(case.q0, case.target)
+
+    def _check(self, rc, what):
+        if rc < 0 or rc > 0:
+            raise AjetError(rc, f"{what}: {lib().ajet_last_error(self.h).decode()}")
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ajet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, q, target=None):
+        self._check(lib().ajet_set_state(self.h, F(q), None if target is None else F(target).ctypes.data),
+                    "ajet_set_state")
+
+    def state(self):
+        out = np.zeros((self.nz, self.nr, 5), order="F")
+        self._check(lib().ajet_get_state(self.h, out), "ajet_get_state")
+        return out
+
+    def compute_dt(self, sync=True):
+        d = C.c_double()
+        self._check(lib().ajet_compute_dt(self.h, C.byref(d) if sync else None), "ajet_compute_dt")
+        return d.value if sync else None
+
+    def rk4_step(self):
+        self._check(lib().ajet_rk4_step(self.h), "ajet_rk4_step")
+
+    def rk4_stage(self, stage):
+        self._check(lib().ajet_rk4_stage(self.h, int(stage)), "ajet_rk4_stage")
+
+    def run(self, nsteps):
+        self._check(lib().ajet_run(self.h, int(nsteps)), "ajet_run")
+
+    def dt_log(self, n=None):
+        t = C.c_double()
+        cnt = lib().ajet_get_dt_log(self.h, None, 0, C.byref(t))
+        if cnt < 0:
+            self._check(cnt, "ajet_get_dt_log")
+        n = cnt if n is None else min(n, cnt)
+        out = np.zeros(max(n, 1))
+        lib().ajet_get_dt_log(self.h, out.ctypes.data, n, None)
+        return out[:n], t.value
+
+    def rhs(self, q):
+        out = np.zeros((self.nz, self.nr, 5), order="F")
+        self._check(lib().ajet_rhs(self.h, F(q), out), "ajet_rhs")
+        return out
+
+    def stream(self):
+        return lib().ajet_stream(self.h)

@@ -1,0 +1,369 @@
+// C ABI of the MI355X RK4 time step (include/ajet_capi.h): context, device buffers in HBM,
+// stream-ordered launches.  The step never needs the host: the time step, the stale
+// temperature of the CFL rule and the accumulated time live on the device.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "ajet_internal.h"
+#include "ajet_kernels.h"
+
+using namespace ajet;
+
+struct ajet_ctx {
+  ajet_params p{};
+  Setup s;
+  Geometry g{};
+  PhysConst pc{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  double *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr, *A = nullptr, *S = nullptr, *tgt = nullptr, *lad = nullptr;
+  uint32_t* meta = nullptr;
+  double *zc = nullptr, *rc = nullptr, *dxidz = nullptr, *ddxidxidz = nullptr, *detadr = nullptr, *ddetadetadr = nullptr;
+  PatchDev* patches = nullptr;
+  uint8_t* seq = nullptr;
+  int *zcuts = nullptr, *rcuts = nullptr;
+  StepScalars* sc = nullptr;
+  double* dtlog = nullptr;
+  long long logcap = 1 << 20;
+  bool dt_valid = false;     // sc->dt holds a time step
+  bool fused_valid = false;  // sc->dmax_bits holds the CFL reduction of the current state
+  bool have_state = false;
+  std::string err;
+};
+
+static const char* kVersion = "axisymmetricjet_amd 0.1 (gfx950, FP64)";
+
+static int fail(ajet_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+static int hip_fail(ajet_ctx* c, hipError_t e, const char* what) {
+  if (c) c->err = std::string(what) + ": " + hipGetErrorString(e);
+  return (int)e;
+}
+
+#define HIPCHK(c, expr)                                   \
+  do {                                                    \
+    hipError_t e_ = (expr);                               \
+    if (e_ != hipSuccess) return hip_fail((c), e_, #expr); \
+  } while (0)
+
+extern "C" {
+
+const char* ajet_version(void) { return kVersion; }
+
+void ajet_params_default(ajet_params* p, int32_t nz, int32_t nr) {
+  if (!p) return;
+  std::memset(p, 0, sizeof *p);
+  p->nz = nz;
+  p->nr = nr;
+  p->viscous = 1;          // do_viscous_terms (ModuleVariables.f90:71)
+  p->shock_capturing = 0;  // ShockCapturing (ModuleVariables.f90:152)
+  p->forcing = 0;          // ControlForcing (ModuleVariables.f90:185)
+  p->gam = 1.4;            // ModuleVariables.f90:66
+  p->Re = 1000.0;          // ModuleVariables.f90:73
+  p->Pr = 0.72;
+  p->CFL = 0.5;            // ModuleVariables.f90:177
+  p->C_mu = 0.0;           // ModuleVariables.f90:164
+  p->C_beta = 0.0;
+  p->C_kappa = 0.0;
+  p->sponge_start_top = 12.5;  // ModuleVariables.f90:12-13
+  p->sponge_A_top = 0.5;
+  p->sponge_end_left = 6.0;
+  p->sponge_A_left = 5.0;
+}
+
+int ajet_build_patches(int32_t nz, int32_t nr, const ajet_bc* lines, int32_t nlines, ajet_patch* out,
+                       int32_t max_out) {
+  if ((!lines && nlines > 0) || nlines < 0 || nz < 1 || nr < 1) return AJET_EINVAL;
+  std::vector<ajet_patch> v;
+  const int n = build_patches(nz, nr, lines, nlines, v);
+  if (n < 0) return n;
+  for (int k = 0; k < n && k < max_out; ++k) out[k] = v[k];
+  return n;
+}
+
+const char* ajet_last_error(const ajet_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void* ajet_stream(ajet_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int ajet_destroy(ajet_ctx* c) {
+  if (!c) return AJET_EINVAL;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void* bufs[] = {c->Q0, c->Q1, c->Q2, c->A, c->S, c->tgt, c->lad, c->meta, c->zc, c->rc, c->dxidz,
+                  c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
+                  c->sc, c->dtlog};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return AJET_OK;
+}
+
+int ajet_create(const ajet_params* p, const double* z, const double* r, const int32_t* iblank,
+                const ajet_bc* lines, int32_t nlines, ajet_ctx** out) {
+  if (!out) return AJET_EINVAL;
+  *out = nullptr;
+  if (!p || !z || !r || (!lines && nlines > 0) || nlines < 0) return AJET_EINVAL;
+  ajet_ctx* c = new ajet_ctx();
+  c->p = *p;
+  int rc = build_setup(*p, z, r, iblank, lines, nlines, c->s, c->err);
+  if (rc == AJET_OK) rc = build_tiling(c->s, p->nz, p->nr, c->err);
+  if (rc == AJET_OK && p->forcing != 0) rc = fail(c, AJET_EUNSUPPORTED, "inflow forcing is not available");
+  if (rc == AJET_OK && p->shock_capturing == 1 && (p->C_mu != 0.0 || p->C_beta != 0.0 || p->C_kappa != 0.0))
+    rc = fail(c, AJET_EUNSUPPORTED, "artificial diffusivity with non-zero constants is not available yet");
+  int ndev = 0;
+  if (rc == AJET_OK && (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1))
+    rc = fail(c, AJET_ENODEV, "no HIP device");
+  if (rc == AJET_OK) {
+    (void)hipGetDevice(&c->device);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess || std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      rc = fail(c, AJET_ENODEV, std::string("device is not gfx950: ") + prop.gcnArchName);
+  }
+  if (rc != AJET_OK) {
+    std::fprintf(stderr, "ajet_create: %s\n", c->err.c_str());
+    *out = c;  // so ajet_last_error can report; the caller destroys it
+    return rc;
+  }
+  const int nz = p->nz, nr = p->nr;
+  Geometry& g = c->g;
+  g.nz = nz;
+  g.nr = nr;
+  g.pitch = (nz + 15) / 16 * 16;
+  g.fstride = (long long)g.pitch * nr;
+  g.ntz = (int)c->s.zcuts.size() - 1;
+  g.ntr = (int)c->s.rcuts.size() - 1;
+  g.any_blank = c->s.any_blank;
+  g.iT = c->s.iT;
+  g.jT = c->s.jT;
+  g.nseq = c->s.nseq;
+  c->pc.gam = p->gam;
+  c->pc.Re = p->Re;
+  c->pc.Pr = p->Pr;
+  c->pc.sponge_start_top = p->sponge_start_top;
+  c->pc.sponge_A_top = p->sponge_A_top;
+  c->pc.sponge_end_left = p->sponge_end_left;
+  c->pc.sponge_A_left = p->sponge_A_left;
+  c->pc.r_top = c->s.rc[nr - 1];
+
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  const size_t sbytes = sizeof(double) * 5 * (size_t)g.fstride;
+  for (double** b : {&c->Q0, &c->Q1, &c->Q2, &c->A, &c->S, &c->tgt}) {
+    HIPCHK(c, hipMalloc((void**)b, sbytes));
+    HIPCHK(c, hipMemsetAsync(*b, 0, sbytes, c->stream));
+  }
+  if (p->shock_capturing == 1) {
+    HIPCHK(c, hipMalloc((void**)&c->lad, sizeof(double) * 3 * (size_t)g.fstride));
+    HIPCHK(c, hipMemsetAsync(c->lad, 0, sizeof(double) * 3 * (size_t)g.fstride, c->stream));
+  }
+  std::vector<uint32_t> meta((size_t)g.fstride, ROW_BLANK | (ROW_BLANK << 4));
+  for (int j = 0; j < nr; ++j)
+    for (int i = 0; i < nz; ++i) meta[(size_t)i + (size_t)g.pitch * j] = c->s.meta[(size_t)i + (size_t)nz * j];
+  HIPCHK(c, hipMalloc((void**)&c->meta, sizeof(uint32_t) * meta.size()));
+  HIPCHK(c, hipMemcpyAsync(c->meta, meta.data(), sizeof(uint32_t) * meta.size(), hipMemcpyHostToDevice, c->stream));
+  struct V { double** d; const std::vector<double>* h; } vs[] = {
+      {&c->zc, &c->s.zc}, {&c->rc, &c->s.rc}, {&c->dxidz, &c->s.dxidz}, {&c->ddxidxidz, &c->s.ddxidxidz},
+      {&c->detadr, &c->s.detadr}, {&c->ddetadetadr, &c->s.ddetadetadr}};
+  for (auto& v : vs) {
+    HIPCHK(c, hipMalloc((void**)v.d, sizeof(double) * v.h->size()));
+    HIPCHK(c, hipMemcpyAsync(*v.d, v.h->data(), sizeof(double) * v.h->size(), hipMemcpyHostToDevice, c->stream));
+  }
+  std::vector<PatchDev> pd(MAX_PATCH);
+  for (size_t k = 0; k < c->s.patches.size(); ++k) {
+    const ajet_patch& P = c->s.patches[k];
+    pd[k] = PatchDev{(int32_t)P.bc, (int32_t)P.bc_dir, (int32_t)P.i_start, (int32_t)P.i_end, (int32_t)P.j_start,
+                     (int32_t)P.j_end};
+  }
+  HIPCHK(c, hipMalloc((void**)&c->patches, sizeof(PatchDev) * MAX_PATCH));
+  HIPCHK(c, hipMemcpyAsync(c->patches, pd.data(), sizeof(PatchDev) * MAX_PATCH, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMalloc((void**)&c->seq, c->s.seq.size()));
+  HIPCHK(c, hipMemcpyAsync(c->seq, c->s.seq.data(), c->s.seq.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMalloc((void**)&c->zcuts, sizeof(int) * c->s.zcuts.size()));
+  HIPCHK(c, hipMemcpyAsync(c->zcuts, c->s.zcuts.data(), sizeof(int) * c->s.zcuts.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMalloc((void**)&c->rcuts, sizeof(int) * c->s.rcuts.size()));
+  HIPCHK(c, hipMemcpyAsync(c->rcuts, c->s.rcuts.data(), sizeof(int) * c->s.rcuts.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMalloc((void**)&c->sc, sizeof(StepScalars)));
+  HIPCHK(c, hipMemsetAsync(c->sc, 0, sizeof(StepScalars), c->stream));
+  HIPCHK(c, hipMalloc((void**)&c->dtlog, sizeof(double) * (size_t)c->logcap));
+  double A[9][6], B[9][6], F[9][11];
+  sbp_tables(A, B, F);
+  upload_tables(A, B, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *out = c;
+  return AJET_OK;
+}
+
+static void copy_state_in(ajet_ctx* c, double* dst, const double* src) {
+  (void)hipMemcpy2DAsync(dst, sizeof(double) * c->g.pitch, src, sizeof(double) * c->g.nz, sizeof(double) * c->g.nz,
+                         (size_t)c->g.nr * 5, hipMemcpyHostToDevice, c->stream);
+}
+
+int ajet_set_state(ajet_ctx* c, const double* pv, const double* pv_target) {
+  if (!c || !c->stream || !pv) return AJET_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  copy_state_in(c, c->Q0, pv);
+  if (pv_target) copy_state_in(c, c->tgt, pv_target);
+  StepScalars z{};
+  HIPCHK(c, hipMemcpyAsync(c->sc, &z, sizeof z, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->dt_valid = false;
+  c->fused_valid = false;
+  c->have_state = true;
+  return AJET_OK;
+}
+
+int ajet_get_state(ajet_ctx* c, double* pv) {
+  if (!c || !c->stream || !pv) return AJET_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpy2DAsync(pv, sizeof(double) * c->g.nz, c->Q0, sizeof(double) * c->g.pitch,
+                             sizeof(double) * c->g.nz, (size_t)c->g.nr * 5, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AJET_OK;
+}
+
+static StageArgs stage_args(ajet_ctx* c) {
+  StageArgs a{};
+  a.tgt = c->tgt;
+  a.meta = c->meta;
+  a.zc = c->zc;
+  a.dxidz = c->dxidz;
+  a.ddxidxidz = c->ddxidxidz;
+  a.rc = c->rc;
+  a.detadr = c->detadr;
+  a.ddetadetadr = c->ddetadetadr;
+  a.lad = c->lad;
+  a.patches = c->patches;
+  a.seq = c->seq;
+  a.zcuts = c->zcuts;
+  a.rcuts = c->rcuts;
+  a.sc = c->sc;
+  a.g = c->g;
+  a.pc = c->pc;
+  a.viscous = c->p.viscous;
+  a.sc_flag = c->p.shock_capturing == 1;
+  return a;
+}
+
+static int read_dt(ajet_ctx* c, double* dt) {
+  StepScalars h{};
+  HIPCHK(c, hipMemcpyAsync(&h, c->sc, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (dt) *dt = h.dt;
+  if (h.nonfinite) return fail(c, AJET_ENONFINITE, "non-finite value in the time-step reduction");
+  if (h.dt == 100.0) return fail(c, AJET_EBLOWUP, "Time step blow up");
+  return AJET_OK;
+}
+
+int ajet_compute_dt(ajet_ctx* c, double* dt) {
+  if (!c || !c->stream) return AJET_EINVAL;
+  if (!c->have_state) return fail(c, AJET_EINVAL, "no state loaded");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipError_t e;
+  if (c->fused_valid) {
+    e = launch_finalize(c->sc, c->p.CFL, c->g.any_blank, c->dtlog, c->logcap, c->stream);
+  } else {
+    CflArgs a{};
+    a.q = c->Q0;
+    a.meta = c->meta;
+    a.dxidz = c->dxidz;
+    a.detadr = c->detadr;
+    a.sc = c->sc;
+    a.g = c->g;
+    a.Re = c->p.Re;
+    a.Pr = c->p.Pr;
+    e = launch_cfl(a, c->p.CFL, c->g.any_blank, c->dtlog, c->logcap, c->stream);
+  }
+  if (e != hipSuccess) return hip_fail(c, e, "time-step reduction launch");
+  c->fused_valid = false;
+  c->dt_valid = true;
+  if (dt) return read_dt(c, dt);
+  return AJET_OK;
+}
+
+// Buffers of the four stages: q^n in Q0, q(1) in Q1, q(2) in Q2, q(3) in Q1, q^{n+1} in Q0.
+int ajet_rk4_stage(ajet_ctx* c, int32_t stage) {
+  if (!c || !c->stream || stage < 1 || stage > 4) return AJET_EINVAL;
+  if (!c->dt_valid) return fail(c, AJET_EINVAL, "no time step: call ajet_compute_dt first");
+  StageArgs a = stage_args(c);
+  const bool lad = false;
+  a.qn = c->Q0;
+  a.acc_a = c->A;
+  a.acc_s = c->S;
+  switch (stage) {
+    case 1: a.qin = c->Q0; a.qout = c->Q1; break;
+    case 2: a.qin = c->Q1; a.qout = c->Q2; break;
+    case 3: a.qin = c->Q2; a.qout = c->Q1; break;
+    default: a.qin = c->Q1; a.qout = c->Q0; break;
+  }
+  HIPCHK(c, launch_stage(stage, lad, a, c->stream));
+  c->fused_valid = stage == 4;
+  return AJET_OK;
+}
+
+int ajet_rk4_step(ajet_ctx* c) {
+  if (!c || !c->stream) return AJET_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  for (int s = 1; s <= 4; ++s) {
+    const int rc = ajet_rk4_stage(c, s);
+    if (rc) return rc;
+  }
+  return AJET_OK;
+}
+
+int ajet_run(ajet_ctx* c, int32_t nsteps) {
+  if (!c || nsteps < 0) return AJET_EINVAL;
+  for (int32_t k = 0; k < nsteps; ++k) {
+    int rc = ajet_compute_dt(c, nullptr);
+    if (rc) return rc;
+    rc = ajet_rk4_step(c);
+    if (rc) return rc;
+  }
+  return AJET_OK;
+}
+
+int ajet_get_dt_log(ajet_ctx* c, double* dts, int32_t n, double* time) {
+  if (!c || !c->stream) return AJET_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  StepScalars h{};
+  HIPCHK(c, hipMemcpyAsync(&h, c->sc, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const long long have = h.nlog < c->logcap ? h.nlog : c->logcap;
+  if (dts && n > 0) {
+    const long long m = have < n ? have : n;
+    if (m > 0) HIPCHK(c, hipMemcpy(dts, c->dtlog, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost));
+  }
+  if (time) *time = h.time;
+  if (h.nonfinite) return fail(c, AJET_ENONFINITE, "non-finite value in the time-step reduction");
+  return (int)(h.nlog > 0x7fffffff ? 0x7fffffff : h.nlog);
+}
+
+int ajet_rhs(ajet_ctx* c, const double* q, double* rhs) {
+  if (!c || !c->stream || !q || !rhs) return AJET_EINVAL;
+  HIPCHK(c, hipSetDevice(c->device));
+  copy_state_in(c, c->Q2, q);
+  StageArgs a = stage_args(c);
+  a.qin = c->Q2;
+  a.qn = c->Q2;
+  a.rhs_out = c->S;
+  HIPCHK(c, launch_stage(0, false, a, c->stream));
+  HIPCHK(c, hipMemcpy2DAsync(rhs, sizeof(double) * c->g.nz, c->S, sizeof(double) * c->g.pitch,
+                             sizeof(double) * c->g.nz, (size_t)c->g.nr * 5, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AJET_OK;
+}
+
+int ajet_lad_coeffs(ajet_ctx* c, const double* q, double* mustar, double* betastar, double* kappastar) {
+  (void)q; (void)mustar; (void)betastar; (void)kappastar;
+  return fail(c, AJET_EUNSUPPORTED, "artificial diffusivity kernels are not available yet");
+}
+
+}  // extern "C"

@@ -1,0 +1,53 @@
+// Kernel argument blocks and launchers (ajet_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ajet_internal.h"
+#include "ajet_physics.h"
+
+namespace ajet {
+
+struct StageArgs {
+  const double* qin;       // stage input state (5 fields), read with a 2-point halo
+  const double* qn;        // q^n (stages 2, 3)
+  double* qout;            // stage output state
+  double* acc_a;           // A = q^n + k1/6
+  double* acc_s;           // S = k2 (+ k3)
+  double* rhs_out;         // stage 0 only: the right-hand side
+  const double* tgt;       // target state
+  const uint32_t* meta;    // per-point metadata word
+  const double* zc;        // z(i)
+  const double* dxidz;     // (nz)
+  const double* ddxidxidz;
+  const double* rc;        // r(j)
+  const double* detadr;    // (nr)
+  const double* ddetadetadr;
+  const double* lad;       // mu*, beta*, kappa* (3 fields) when shock capturing is on
+  const PatchDev* patches;
+  const uint8_t* seq;
+  const int* zcuts;        // tile boundaries along z (ntz + 1 entries)
+  const int* rcuts;        // tile boundaries along r (ntr + 1 entries)
+  StepScalars* sc;
+  Geometry g;
+  PhysConst pc;
+  int viscous;
+  int sc_flag;             // ShockCapturing == 1 (per-point viscous coefficients)
+};
+
+struct CflArgs {
+  const double* q;
+  const uint32_t* meta;
+  const double* dxidz;
+  const double* detadr;
+  StepScalars* sc;
+  Geometry g;
+  double Re, Pr;
+};
+
+void upload_tables(const double sbp1[9][6], const double sbp2[9][6], hipStream_t s);
+hipError_t launch_stage(int stage, bool lad, const StageArgs& a, hipStream_t s);
+hipError_t launch_cfl(const CflArgs& a, double CFL, int any_blank, double* log, long long cap, hipStream_t s);
+hipError_t launch_finalize(StepScalars* sc, double CFL, int any_blank, double* log, long long cap, hipStream_t s);
+
+}  // namespace ajet

@@ -1,0 +1,403 @@
+// gfx950 kernels of the RK4 time step: the fused stage kernel (ComputeRHS + stage update +
+// axis fix, with the CFL reduction of the next step fused into the last stage), the
+// stand-alone CFL reduction, and its finalisation.  Build with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "ajet_kernels.h"
+#include "ajet_physics.h"
+
+namespace ajet {
+
+__constant__ double c_sbp1[9][6];
+__constant__ double c_sbp2[9][6];
+
+// Interior stencils SBP1(4,0:4) and SBP2(4,0:4) (ModOperator.f90:42, 78).
+constexpr double D1A = 1.0 / 12.0, D1B = -(2.0 / 3.0), D1D = 2.0 / 3.0, D1E = -(1.0 / 12.0);
+constexpr double D2A = -(1.0 / 12.0), D2B = 4.0 / 3.0, D2C = -(5.0 / 2.0), D2D = 4.0 / 3.0, D2E = -(1.0 / 12.0);
+
+// SBP first-derivative stencil sum (before the metric) at a point whose row code is `row`;
+// `c` points at the point itself inside a strided line, stride s.  The interior sum skips
+// the zero centre weight, which adds an exact zero in the reference.
+__device__ __forceinline__ double d1_sum(uint32_t row, const double* c, int s) {
+  if (row == ROW_INTERIOR) return ((D1A * c[-2 * s] + D1B * c[-s]) + D1D * c[s]) + D1E * c[2 * s];
+  if (row == ROW_BLANK) return 0.0;
+  const int base = row < 4 ? -(int)row : -(int)row + 3;
+  const double* v = c + base * s;
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) acc = acc + c_sbp1[row][m] * v[m * s];
+  return acc;
+}
+
+__device__ __forceinline__ double d2_sum(uint32_t row, const double* c, int s) {
+  if (row == ROW_INTERIOR) return (((D2A * c[-2 * s] + D2B * c[-s]) + D2C * c[0]) + D2D * c[s]) + D2E * c[2 * s];
+  if (row == ROW_BLANK) return 0.0;
+  const int base = row < 4 ? -(int)row : -(int)row + 3;
+  const double* v = c + base * s;
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 0; m < 6; ++m) acc = acc + c_sbp2[row][m] * v[m * s];
+  return acc;
+}
+
+// Tile t owns [cuts[t], cuts[t+1]) (at most TZ or TR points); its window starts HALO before.
+__device__ __forceinline__ void tile_range(int t, const int* cuts, int& lo, int& own_lo, int& own_hi) {
+  own_lo = cuts[t];
+  own_hi = cuts[t + 1];
+  lo = own_lo;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous run of
+// tiles (z fastest) so that tiles sharing halo rows meet in the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nblk_pad) { return (b & 7) * (nblk_pad >> 3) + (b >> 3); }
+
+__device__ __forceinline__ double warp_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Denominator of ComputeTimeStep (ModInputOutput.f90:404-412) at one point.
+__device__ __forceinline__ double cfl_denominator(double Vr, double Vz, double T, double sTst, double a, double b,
+                                                  double Re, double Pr) {
+  const double c35 = 3.0 / 5.0, c23 = 2.0 / 3.0;
+  const double mu = pow(T, 0.666);
+  const double beta = c35 * mu;
+  const double lambda = beta - c23 * mu;
+  const double inv = fabs(Vz * a + Vr * b) + sTst * sqrt(a * a + b * b);
+  const double m1 = mu / Re, m2 = lambda / Re, m3 = mu / (Re * Pr);
+  double mx = m1;
+  if (m2 > mx) mx = m2;
+  if (m3 > mx) mx = m3;
+  const double s = fabs(a) + fabs(b);
+  return inv + 2.0 * mx * (s * s);
+}
+
+__device__ __forceinline__ void block_max_atomic(double v, unsigned long long* dst, double* red) {
+  v = warp_max(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmax(m, red[w]);
+    if (m > 0.0) atomicMax(dst, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// NF staged fields: rho, Vr, Vt, Vz, T, p, mu, kappa (kappa only with LAD; it equals mu
+// otherwise, AssignValues ModAssign.f90:22).
+template <int STAGE, bool LAD>
+__global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a) {
+  constexpr int NF = LAD ? 8 : 7;
+  __shared__ double sF[NF][WR][WZ];
+  __shared__ double sD[2][WR][TZ];
+  __shared__ double red[STAGE_THREADS / 64];
+
+  const Geometry& g = a.g;
+  const int nz = g.nz, nr = g.nr, pitch = g.pitch;
+  const long long fs = g.fstride;
+  const int nblk = g.ntz * g.ntr;
+  const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (t >= nblk) return;
+  const int tzi = t % g.ntz, tri = t / g.ntz;
+  int i0, oi_lo, oi_hi, j0, oj_lo, oj_hi;
+  tile_range(tzi, a.zcuts, i0, oi_lo, oi_hi);
+  tile_range(tri, a.rcuts, j0, oj_lo, oj_hi);
+
+  // ---- stage the window: primitives, p = rho T, mu = T**0.666 (AssignValues)
+  for (int k = threadIdx.x; k < WZ * WR; k += STAGE_THREADS) {
+    const int wz = k % WZ, wr = k / WZ;
+    const int i = i0 - HALO + wz, j = j0 - HALO + wr;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, p = 0.0, mu = 0.0, kap = 0.0;
+    if (i >= 0 && i < nz && j >= 0 && j < nr) {
+      const long long gi = (long long)i + (long long)pitch * j;
+#pragma unroll
+      for (int f = 0; f < 5; ++f) v[f] = a.qin[gi + f * fs];
+      p = v[0] * v[4];
+      mu = pow(v[4], 0.666);
+      kap = mu;
+      if (LAD) {
+        mu = mu + a.lad[gi];
+        kap = kap + a.lad[gi + 2 * fs];
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < 5; ++f) sF[f][wr][wz] = v[f];
+    sF[5][wr][wz] = p;
+    sF[6][wr][wz] = mu;
+    if (LAD) sF[NF - 1][wr][wz] = kap;
+  }
+  __syncthreads();
+
+  // ---- dVr/dz and dVz/dz on every window row of the owned columns (for the cross terms)
+  for (int k = threadIdx.x; k < TZ * WR; k += STAGE_THREADS) {
+    const int tz = k % TZ, wr = k / TZ;
+    const int i = i0 + tz, j = j0 - HALO + wr;
+    double dvr = 0.0, dvz = 0.0;
+    if (i >= oi_lo && i < oi_hi && j >= 0 && j < nr) {
+      const uint32_t zr = a.meta[(long long)i + (long long)pitch * j] & 15u;
+      const double m = a.dxidz[i];
+      dvr = d1_sum(zr, &sF[1][wr][tz + HALO], 1) * m;
+      dvz = d1_sum(zr, &sF[3][wr][tz + HALO], 1) * m;
+    }
+    sD[0][wr][tz] = dvr;
+    sD[1][wr][tz] = dvz;
+  }
+  __syncthreads();
+
+  const double dt = (STAGE > 0) ? a.sc->dt : 0.0;
+  double dmax = 0.0;
+  double sTst = 0.0;
+  if (STAGE == 4) {
+    const double Tst = a.qin[(long long)g.iT + (long long)pitch * g.jT + 4 * fs];
+    sTst = sqrt(Tst);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      a.sc->T_stale = Tst;
+      a.sc->time = a.sc->time + dt;
+    }
+  }
+  if (STAGE == 1 && blockIdx.x == 0 && threadIdx.x == 0) a.sc->dmax_bits = 0ull;
+
+  const int tz = threadIdx.x % TZ;
+  for (int ty = threadIdx.x / TZ; ty < TR; ty += STAGE_THREADS / TZ) {
+    const int i = i0 + tz, j = j0 + ty;
+    if (i < oi_lo || i >= oi_hi || j < oj_lo || j >= oj_hi) continue;
+    const int wz = tz + HALO, wr = ty + HALO;
+    const long long gi = (long long)i + (long long)pitch * j;
+    const uint32_t meta = a.meta[gi];
+    const uint32_t zr = meta & 15u, rr = (meta >> 4) & 15u;
+    const double mz = a.dxidz[i], mz2 = a.ddxidxidz[i];
+    const double mr = a.detadr[j], mr2 = a.ddetadetadr[j];
+
+    double q[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) q[f] = sF[f][wr][wz];
+
+    PointDerivs d;
+    d.drhodz = d1_sum(zr, &sF[0][wr][wz], 1) * mz;
+    d.dVrdz = sD[0][wr][tz];
+    d.dVtdz = d1_sum(zr, &sF[2][wr][wz], 1) * mz;
+    d.dVzdz = sD[1][wr][tz];
+    d.dTdz = d1_sum(zr, &sF[4][wr][wz], 1) * mz;
+    d.dpdz = d1_sum(zr, &sF[5][wr][wz], 1) * mz;
+    d.drhodr = d1_sum(rr, &sF[0][wr][wz], WZ) * mr;
+    d.dVrdr = d1_sum(rr, &sF[1][wr][wz], WZ) * mr;
+    d.dVtdr = d1_sum(rr, &sF[2][wr][wz], WZ) * mr;
+    d.dVzdr = d1_sum(rr, &sF[3][wr][wz], WZ) * mr;
+    d.dTdr = d1_sum(rr, &sF[4][wr][wz], WZ) * mr;
+    d.dpdr = d1_sum(rr, &sF[5][wr][wz], WZ) * mr;
+
+    double rhs[5];
+    const double r = a.rc[j];
+    if (j == 0) {
+      // the axis row is skipped by the interior loops (ModRHS.f90:59-67): rhs = -0
+#pragma unroll
+      for (int k = 0; k < 5; ++k) rhs[k] = -1.0 * 0.0;
+    } else {
+      inviscid(q, r, d, a.pc.gam, rhs);
+    }
+    double mu = sF[6][wr][wz];
+    if (a.viscous) {
+      d.dmudz = d1_sum(zr, &sF[6][wr][wz], 1) * mz;
+      d.dmudr = d1_sum(rr, &sF[6][wr][wz], WZ) * mr;
+      double kappa = mu;
+      if (LAD) {
+        kappa = sF[NF - 1][wr][wz];
+        d.dkappadz = d1_sum(zr, &sF[NF - 1][wr][wz], 1) * mz;
+        d.dkappadr = d1_sum(rr, &sF[NF - 1][wr][wz], WZ) * mr;
+      } else {
+        d.dkappadz = d.dmudz;
+        d.dkappadr = d.dmudr;
+      }
+      const double z2 = mz * mz, r2 = mr * mr;
+      d.d2Vrdz2 = d2_sum(zr, &sF[1][wr][wz], 1) * z2 + d.dVrdz * mz2;
+      d.d2Vtdz2 = d2_sum(zr, &sF[2][wr][wz], 1) * z2 + d.dVtdz * mz2;
+      d.d2Vzdz2 = d2_sum(zr, &sF[3][wr][wz], 1) * z2 + d.dVzdz * mz2;
+      d.d2Tdz2 = d2_sum(zr, &sF[4][wr][wz], 1) * z2 + d.dTdz * mz2;
+      d.d2Vrdr2 = d2_sum(rr, &sF[1][wr][wz], WZ) * r2 + d.dVrdr * mr2;
+      d.d2Vtdr2 = d2_sum(rr, &sF[2][wr][wz], WZ) * r2 + d.dVtdr * mr2;
+      d.d2Vzdr2 = d2_sum(rr, &sF[3][wr][wz], WZ) * r2 + d.dVzdr * mr2;
+      d.d2Tdr2 = d2_sum(rr, &sF[4][wr][wz], WZ) * r2 + d.dTdr * mr2;
+      d.d2Vrdzdr = d1_sum(rr, &sD[0][wr][tz], TZ) * mr;
+      d.d2Vzdzdr = d1_sum(rr, &sD[1][wr][tz], TZ) * mr;
+      if (j != 0) {
+        double lambda = 0.0;
+        if (a.sc_flag) {
+          const double mu_phys = pow(q[4], 0.666);
+          double beta = (3.0 / 5.0) * mu_phys;
+          if (LAD) beta = beta + a.lad[gi + fs];
+          lambda = beta - (2.0 / 3.0) * mu;
+        }
+        viscous(q, r, mu, kappa, lambda, a.sc_flag != 0, d, a.pc, rhs);
+      }
+    }
+
+    // boundary conditions (ModBC.f90:16-277), patches in bc-file order, then the sponges
+    const uint32_t sid = (meta >> 8) & 255u;
+    const double zc = a.zc[i];
+    const bool top = r >= a.pc.sponge_start_top;
+    const bool inflow = j >= 1 && zc <= a.pc.sponge_end_left;
+    if (sid != 0 || top || inflow) {
+      double qt[5];
+#pragma unroll
+      for (int f = 0; f < 5; ++f) qt[f] = a.tgt[gi + f * fs];
+      if (sid != 0) {
+        for (int s = 0; s < SEQ_LEN; ++s) {
+          const int pidx = a.seq[sid * SEQ_LEN + s];
+          if (pidx == 0) break;
+          const PatchDev P = a.patches[pidx - 1];
+          apply_patch(P.bc, P.dir, q, qt, d, mz, mr, a.pc.gam, rhs);
+        }
+      }
+      if (top) {
+        const double zeta = (r - a.pc.sponge_start_top) / (a.pc.r_top - a.pc.sponge_start_top);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rhs[k] = rhs[k] - a.pc.sponge_A_top * (zeta * zeta) * (q[k] - qt[k]);
+      }
+      if (inflow) {
+        const double zeta = (a.pc.sponge_end_left - zc) / a.pc.sponge_end_left;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rhs[k] = rhs[k] - a.pc.sponge_A_left * (zeta * zeta) * (q[k] - qt[k]);
+      }
+    }
+
+    if (STAGE == 0) {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) a.rhs_out[gi + k * fs] = rhs[k];
+      continue;
+    }
+
+    // ---- stage update (ModRungeKutta.f90:32-146)
+    double out[5];
+    const double c16 = 1.0 / 6.0, c13 = 1.0 / 3.0;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      const double kk = dt * rhs[k];
+      if (STAGE == 1) {
+        out[k] = q[k] + 0.5 * kk;
+        a.acc_a[gi + k * fs] = q[k] + c16 * kk;
+      } else if (STAGE == 2) {
+        out[k] = a.qn[gi + k * fs] + 0.5 * kk;
+        a.acc_s[gi + k * fs] = kk;
+      } else if (STAGE == 3) {
+        out[k] = a.qn[gi + k * fs] + kk;
+        a.acc_s[gi + k * fs] = a.acc_s[gi + k * fs] + kk;
+      } else {
+        out[k] = (a.acc_a[gi + k * fs] + c13 * a.acc_s[gi + k * fs]) + c16 * kk;
+      }
+    }
+    if (STAGE == 4 && zr == ROW_BLANK) {
+      // blanked points skip the final update and keep the last stage value (ModRungeKutta.f90:139)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) out[k] = q[k];
+    }
+    if (j == 0) {
+      // axis_conditions (ModAxis.f90:14-16) from the stage input rows 2..6
+      const double a00 = c_sbp1[0][0];
+      const int fl[3] = {0, 3, 4};
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        const int f = fl[u];
+        const double s = (c_sbp1[0][1] * sF[f][wr + 1][wz] + c_sbp1[0][2] * sF[f][wr + 2][wz]) +
+                         c_sbp1[0][3] * sF[f][wr + 3][wz];
+        out[f] = -1.0 * s / a00;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) a.qout[gi + k * fs] = out[k];
+
+    if (STAGE == 4 && zr != ROW_BLANK) {
+      const double den = cfl_denominator(out[1], out[3], out[4], sTst, mz, mr, a.pc.Re, a.pc.Pr);
+      if (!(den <= 1.0e300)) a.sc->nonfinite = 1;
+      dmax = fmax(dmax, den);
+    }
+  }
+  if (STAGE == 4) block_max_atomic(dmax, &a.sc->dmax_bits, red);
+}
+
+// ComputeTimeStep's sweep (ModInputOutput.f90:395-416) over the current state with the
+// stale scalar T kept on the device.
+__global__ __launch_bounds__(256) void cfl_kernel(CflArgs a) {
+  __shared__ double red[4];
+  const Geometry& g = a.g;
+  const double sTst = sqrt(a.sc->T_stale);
+  double dmax = 0.0;
+  const long long n = (long long)g.nz * g.nr;
+  for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(p % g.nz), j = (int)(p / g.nz);
+    const long long gi = (long long)i + (long long)g.pitch * j;
+    if ((a.meta[gi] & 15u) == ROW_BLANK) continue;
+    const double den = cfl_denominator(a.q[gi + g.fstride], a.q[gi + 3 * g.fstride], a.q[gi + 4 * g.fstride], sTst,
+                                       a.dxidz[i], a.detadr[j], a.Re, a.Pr);
+    if (!(den <= 1.0e300)) a.sc->nonfinite = 1;
+    dmax = fmax(dmax, den);
+  }
+  block_max_atomic(dmax, &a.sc->dmax_bits, red);
+}
+
+__global__ void scalar_reset_kernel(StepScalars* sc) {
+  if (threadIdx.x == 0) sc->dmax_bits = 0ull;
+}
+
+// dt = MINVAL(CFL / denominator) = CFL / MAXVAL(denominator), since a correctly rounded
+// division by a positive number is monotone; blanked points contribute 100
+// (ModInputOutput.f90:399-402).
+__global__ void cfl_finalize_kernel(StepScalars* sc, double* log, long long log_cap, double CFL, int any_blank) {
+  if (threadIdx.x != 0) return;
+  const unsigned long long b = sc->dmax_bits;
+  double dt = b == 0ull ? 100.0 : CFL / __longlong_as_double((long long)b);
+  if (any_blank && dt > 100.0) dt = 100.0;
+  sc->dt = dt;
+  if (sc->nlog < log_cap) log[sc->nlog] = dt;
+  sc->nlog = sc->nlog + 1;
+}
+
+void upload_tables(const double sbp1[9][6], const double sbp2[9][6], hipStream_t s) {
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sbp1), sbp1, sizeof(double) * 54, 0, hipMemcpyHostToDevice, s);
+  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sbp2), sbp2, sizeof(double) * 54, 0, hipMemcpyHostToDevice, s);
+}
+
+static int pad8(int n) { return (n + 7) / 8 * 8; }
+
+hipError_t launch_stage(int stage, bool lad, const StageArgs& a, hipStream_t s) {
+  const int nblk = pad8(a.g.ntz * a.g.ntr);
+  dim3 grid(nblk), block(STAGE_THREADS);
+#define AJET_LAUNCH(S, L) hipLaunchKernelGGL((stage_kernel<S, L>), grid, block, 0, s, a)
+  if (!lad) {
+    switch (stage) {
+      case 0: AJET_LAUNCH(0, false); break;
+      case 1: AJET_LAUNCH(1, false); break;
+      case 2: AJET_LAUNCH(2, false); break;
+      case 3: AJET_LAUNCH(3, false); break;
+      default: AJET_LAUNCH(4, false); break;
+    }
+  } else {
+    switch (stage) {
+      case 0: AJET_LAUNCH(0, true); break;
+      case 1: AJET_LAUNCH(1, true); break;
+      case 2: AJET_LAUNCH(2, true); break;
+      case 3: AJET_LAUNCH(3, true); break;
+      default: AJET_LAUNCH(4, true); break;
+    }
+  }
+#undef AJET_LAUNCH
+  return hipGetLastError();
+}
+
+hipError_t launch_cfl(const CflArgs& a, double CFL, int any_blank, double* log, long long cap, hipStream_t s) {
+  hipLaunchKernelGGL(scalar_reset_kernel, dim3(1), dim3(64), 0, s, a.sc);
+  const long long n = (long long)a.g.nz * a.g.nr;
+  int nb = (int)((n + 255) / 256);
+  if (nb > 2048) nb = 2048;
+  hipLaunchKernelGGL(cfl_kernel, dim3(nb), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(cfl_finalize_kernel, dim3(1), dim3(64), 0, s, a.sc, log, cap, CFL, any_blank);
+  return hipGetLastError();
+}
+
+hipError_t launch_finalize(StepScalars* sc, double CFL, int any_blank, double* log, long long cap, hipStream_t s) {
+  hipLaunchKernelGGL(cfl_finalize_kernel, dim3(1), dim3(64), 0, s, sc, log, cap, CFL, any_blank);
+  return hipGetLastError();
+}
+
+}  // namespace ajet

@@ -110,6 +110,10 @@ int ajet_compute_dt(ajet_ctx* ctx, double* dt);
  * time step of the last ajet_compute_dt (or of the previous step's fused reduction). */
 int ajet_rk4_step(ajet_ctx* ctx);
 
+/* One of the four RK stages of ajet_rk4_step (stage = 1..4), launched alone so that a caller
+ * can time the stage kernel; calling stages 1, 2, 3, 4 in order is one ajet_rk4_step. */
+int ajet_rk4_stage(ajet_ctx* ctx, int32_t stage);
+
 /* nsteps iterations of {ComputeTimeStep; RK4} (ModSolver.f90:122-125).  The time step of
  * each iteration is reduced on the device inside the previous step's last stage; the
  * sequence is appended to the device time-step log (see ajet_get_dt_log). */

@@ -1,0 +1,112 @@
+"""GPU parity of the HIP path against the oracle and the reference's golden fixtures.
+
+Every call goes through the C ABI (libajet.so).  Tolerances: the HIP kernels keep the
+reference's operation order with no fused multiply-add, so the only differences come from
+the device pow (T**0.666) rounding differently from glibc's by at most an ulp or so; the
+bars are the north-star ones -- a single RHS within 1e-12 (normwise per equation,
+max|gpu - ref| / max|ref|) and the state within 1e-9 (max norm) after many steps.
+"""
+import numpy as np
+import pytest
+
+from axisymmetricjet_amd import capi
+from axisymmetricjet_amd import jetcase as jc
+from oracle import pyoracle as po
+import cases as gc
+
+pytestmark = pytest.mark.gpu
+
+RHS_TOL = 1e-12
+STATE_TOL = 1e-9
+
+
+def normwise(a, b):
+    out = []
+    for e in range(5):
+        den = np.abs(b[:, :, e]).max()
+        num = np.abs(a[:, :, e] - b[:, :, e]).max()
+        out.append(num / den if den > 0 else num)
+    return max(out)
+
+
+@pytest.mark.parametrize("name", ["jet88", "jet440", "nozzle96", "walls64", "swirl64", "jet440_lad"])
+def test_first_rhs_vs_reference(name):
+    case, g, shock, nsteps, extra = gc.load(name)
+    print(f"case {name} shock={shock}", flush=True)
+    ctx = capi.Context(case, shock_capturing=shock)
+    got = ctx.rhs(case.q0)
+    err = normwise(got, g["rhs"])
+    print(f"{name}: rhs normwise rel err {err:.3e}", flush=True)
+    assert err <= RHS_TOL
+
+
+@pytest.mark.parametrize("name", ["jet88", "nozzle96", "walls64", "swirl64"])
+def test_trajectory_vs_reference(name):
+    case, g, shock, nsteps, extra = gc.load(name)
+    print(f"case {name} steps={nsteps}", flush=True)
+    ctx = capi.Context(case, shock_capturing=shock)
+    ctx.run(nsteps)
+    dts, t = ctx.dt_log()
+    st = ctx.state()
+    assert len(dts) == nsteps
+    np.testing.assert_allclose(dts, g["dts"], rtol=1e-12, atol=0)
+    assert abs(t - float(g["time"])) <= 1e-12 * abs(float(g["time"]))
+    err = np.abs(st - g["state"]).max()
+    print(f"{name}: state max-norm err {err:.3e}", flush=True)
+    assert err <= STATE_TOL
+
+
+def test_jet440_100_steps_vs_reference():
+    """configs[0]-sized run: 440 x 64, 100 RK4 steps, against the reference's own trajectory."""
+    for name in ("jet440", "jet440_lad"):
+        case, g, shock, nsteps, extra = gc.load(name)
+        print(f"case {name} steps={nsteps}", flush=True)
+        ctx = capi.Context(case, shock_capturing=shock)
+        ctx.run(nsteps)
+        dts, t = ctx.dt_log()
+        np.testing.assert_allclose(dts, g["dts"], rtol=1e-12, atol=0)
+        err = np.abs(ctx.state() - g["state"]).max()
+        print(f"{name}: state max-norm err after {nsteps} steps {err:.3e}", flush=True)
+        assert err <= STATE_TOL
+
+
+def test_step_by_step_api_matches_run():
+    case = jc.make_jet(96, 48, seed=21)
+    a = capi.Context(case)
+    b = capi.Context(case)
+    a.run(5)
+    for _ in range(5):
+        b.compute_dt()
+        b.rk4_step()
+    assert np.array_equal(a.state(), b.state())
+    assert np.array_equal(a.dt_log()[0], b.dt_log()[0])
+
+
+def test_rhs_vs_oracle_random_state():
+    """A random perturbation of the jet (swirl on) through the full BC set, vs the oracle."""
+    case = jc.make_nozzle(160, 96, i_lip=40, j_in=24, j_out=30, seed=5)
+    rng = np.random.default_rng(3)
+    q = np.array(case.q0, order="F")
+    q[:, :, 2] += 1e-3 * rng.standard_normal(q.shape[:2])
+    o = po.Oracle(case)
+    ctx = capi.Context(case)
+    ref = o.rhs(q)
+    got = ctx.rhs(q)
+    assert normwise(got, ref) <= RHS_TOL
+
+
+def test_rhs_does_not_change_state():
+    case = jc.make_jet(64, 32, seed=2)
+    ctx = capi.Context(case)
+    before = ctx.state()
+    ctx.rhs(case.q0 * 1.01)
+    assert np.array_equal(before, ctx.state())
+
+
+def test_blowup_reported():
+    case = jc.make_jet(64, 32, seed=2)
+    case.iblank[:] = 0
+    ctx = capi.Context(case)
+    with pytest.raises(capi.AjetError) as ei:
+        ctx.compute_dt()
+    assert ei.value.code == -5
