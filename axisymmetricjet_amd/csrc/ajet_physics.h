@@ -138,7 +138,7 @@ __device__ __forceinline__ void tlam_w(const Eig& e, bool rdir, const double lam
 
 // One patch of BoundaryConditions (ModBC.f90:54-267) at one point.  q is the stage state,
 // qt the target; met is dxidz for |dir| = 1 and detadr for |dir| = 2.
-__device__ __noinline__ void apply_patch(int bc, int dir, const double q[5], const double qt[5],
+__device__ __forceinline__ void apply_patch(int bc, int dir, const double q[5], const double qt[5],
                                          const PointDerivs& d, double dxidz, double detadr, double gam,
                                          double rhs[5]) {
   const double sigma_1 = -2.0, sigma_2 = -2.0, h44 = 48.0 / 49.0;

@@ -59,6 +59,9 @@ def main(argv=None):
     ap.add_argument("--stage-steps", type=int, default=20, help="steps timed launch by launch for the roofline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run one device copy of this many MiB (a known-size kernel that scales "
+                         "hardware byte counters)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (use --gpus 1)",
@@ -68,6 +71,15 @@ def main(argv=None):
     import torch
     from axisymmetricjet_amd import capi
     from axisymmetricjet_amd import jetcase as jc
+
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calib_copy_mib} MiB read + {args.calib_copy_mib} MiB written")
+        del src, dst
 
     nz, nr = args.nz, args.nr
     npts = nz * nr
