@@ -26,6 +26,8 @@ struct ajet_ctx {
   PatchDev* patches = nullptr;
   uint8_t* seq = nullptr;
   int *zcuts = nullptr, *rcuts = nullptr;
+  int *int_tiles = nullptr, *bnd_tiles = nullptr;  // interior and boundary tile lists
+  int n_int = 0, n_bnd = 0;
   StepScalars* sc = nullptr;
   double* dtlog = nullptr;
   long long logcap = 1 << 20;
@@ -98,7 +100,7 @@ int ajet_destroy(ajet_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* bufs[] = {c->Q0, c->Q1, c->Q2, c->A, c->S, c->tgt, c->lad, c->meta, c->zc, c->rc, c->dxidz,
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
-                  c->sc, c->dtlog};
+                  c->int_tiles, c->bnd_tiles, c->sc, c->dtlog};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -191,6 +193,18 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   HIPCHK(c, hipMalloc((void**)&c->rcuts, sizeof(int) * c->s.rcuts.size()));
   HIPCHK(c, hipMemcpyAsync(c->rcuts, c->s.rcuts.data(), sizeof(int) * c->s.rcuts.size(), hipMemcpyHostToDevice,
                            c->stream));
+  {
+    std::vector<int> it, bt;
+    classify_tiles(c->s, nz, nr, it, bt);
+    c->n_int = (int)it.size();
+    c->n_bnd = (int)bt.size();
+    HIPCHK(c, hipMalloc((void**)&c->int_tiles, sizeof(int) * (it.size() + 1)));
+    HIPCHK(c, hipMalloc((void**)&c->bnd_tiles, sizeof(int) * (bt.size() + 1)));
+    if (!it.empty())
+      HIPCHK(c, hipMemcpy(c->int_tiles, it.data(), sizeof(int) * it.size(), hipMemcpyHostToDevice));
+    if (!bt.empty())
+      HIPCHK(c, hipMemcpy(c->bnd_tiles, bt.data(), sizeof(int) * bt.size(), hipMemcpyHostToDevice));
+  }
   HIPCHK(c, hipMalloc((void**)&c->sc, sizeof(StepScalars)));
   HIPCHK(c, hipMemsetAsync(c->sc, 0, sizeof(StepScalars), c->stream));
   HIPCHK(c, hipMalloc((void**)&c->dtlog, sizeof(double) * (size_t)c->logcap));
@@ -253,6 +267,25 @@ static StageArgs stage_args(ajet_ctx* c) {
   return a;
 }
 
+// One stage over the whole grid: the boundary tiles (general kernel; it also updates the
+// step scalars) and then the interior tiles (specialised kernel).
+static int physics_mode(const ajet_params& p) {
+  if (p.viscous != 1) return MODE_INVISCID;
+  if (p.shock_capturing != 1) return MODE_PLAIN;
+  return (p.C_mu != 0.0 || p.C_beta != 0.0 || p.C_kappa != 0.0) ? MODE_LAD : MODE_SC;
+}
+
+static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
+  const int mode = physics_mode(c->p);
+  a.tiles = c->bnd_tiles;
+  a.ntiles = c->n_bnd;
+  hipError_t e = launch_stage(stage, mode, false, a, c->stream);
+  if (e != hipSuccess) return e;
+  a.tiles = c->int_tiles;
+  a.ntiles = c->n_int;
+  return launch_stage(stage, mode, true, a, c->stream);
+}
+
 static int read_dt(ajet_ctx* c, double* dt) {
   StepScalars h{};
   HIPCHK(c, hipMemcpyAsync(&h, c->sc, sizeof h, hipMemcpyDeviceToHost, c->stream));
@@ -294,7 +327,6 @@ int ajet_rk4_stage(ajet_ctx* c, int32_t stage) {
   if (!c || !c->stream || stage < 1 || stage > 4) return AJET_EINVAL;
   if (!c->dt_valid) return fail(c, AJET_EINVAL, "no time step: call ajet_compute_dt first");
   StageArgs a = stage_args(c);
-  const bool lad = false;
   a.qn = c->Q0;
   a.acc_a = c->A;
   a.acc_s = c->S;
@@ -304,7 +336,7 @@ int ajet_rk4_stage(ajet_ctx* c, int32_t stage) {
     case 3: a.qin = c->Q2; a.qout = c->Q1; break;
     default: a.qin = c->Q1; a.qout = c->Q0; break;
   }
-  HIPCHK(c, launch_stage(stage, lad, a, c->stream));
+  HIPCHK(c, launch_stages(c, stage, a));
   c->fused_valid = stage == 4;
   return AJET_OK;
 }
@@ -354,7 +386,7 @@ int ajet_rhs(ajet_ctx* c, const double* q, double* rhs) {
   a.qin = c->Q2;
   a.qn = c->Q2;
   a.rhs_out = c->S;
-  HIPCHK(c, launch_stage(0, false, a, c->stream));
+  HIPCHK(c, launch_stages(c, 0, a));
   HIPCHK(c, hipMemcpy2DAsync(rhs, sizeof(double) * c->g.nz, c->S, sizeof(double) * c->g.pitch,
                              sizeof(double) * c->g.nz, (size_t)c->g.nr * 5, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -74,5 +74,6 @@ int build_patches(int nz, int nr, const ajet_bc* lines, int nlines, std::vector<
 int build_setup(const ajet_params& p, const double* z, const double* r, const int32_t* iblank,
                 const ajet_bc* lines, int nlines, Setup& s, std::string& err);
 int build_tiling(Setup& s, int nz, int nr, std::string& err);
+void classify_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, std::vector<int>& boundary);
 
 }  // namespace ajet

@@ -28,6 +28,8 @@ struct StageArgs {
   const uint8_t* seq;
   const int* zcuts;        // tile boundaries along z (ntz + 1 entries)
   const int* rcuts;        // tile boundaries along r (ntr + 1 entries)
+  const int* tiles;        // tiles of this launch (id = tri * ntz + tzi)
+  int ntiles;
   StepScalars* sc;
   Geometry g;
   PhysConst pc;
@@ -46,7 +48,10 @@ struct CflArgs {
 };
 
 void upload_tables(const double sbp1[9][6], const double sbp2[9][6], hipStream_t s);
-hipError_t launch_stage(int stage, bool lad, const StageArgs& a, hipStream_t s);
+// Physics variants of the stage kernel.
+enum { MODE_PLAIN = 0, MODE_SC = 1, MODE_LAD = 2, MODE_INVISCID = 3 };
+
+hipError_t launch_stage(int stage, int mode, bool interior, const StageArgs& a, hipStream_t s);
 hipError_t launch_cfl(const CflArgs& a, double CFL, int any_blank, double* log, long long cap, hipStream_t s);
 hipError_t launch_finalize(StepScalars* sc, double CFL, int any_blank, double* log, long long cap, hipStream_t s);
 

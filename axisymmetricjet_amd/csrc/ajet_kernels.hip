@@ -12,6 +12,17 @@ __constant__ double c_sbp1[9][6];
 __constant__ double c_sbp2[9][6];
 
 // Interior stencils SBP1(4,0:4) and SBP2(4,0:4) (ModOperator.f90:42, 78).
+// Three 256-thread blocks fit one CU's LDS; ask the compiler for registers that let them all
+// run (at most 168 VGPRs).
+#ifndef AJET_STAGE_WAVES
+#define AJET_STAGE_WAVES 3
+#endif
+constexpr int STAGE_WAVES_PER_EU = AJET_STAGE_WAVES;
+
+// Scheduling fence: keeps the stencil loads of one derivative group from being hoisted
+// above the previous group, which bounds the registers live at once.
+#define AJET_FENCE() __builtin_amdgcn_sched_barrier(0)
+
 constexpr double D1A = 1.0 / 12.0, D1B = -(2.0 / 3.0), D1D = 2.0 / 3.0, D1E = -(1.0 / 12.0);
 constexpr double D2A = -(1.0 / 12.0), D2B = 4.0 / 3.0, D2C = -(5.0 / 2.0), D2D = 4.0 / 3.0, D2E = -(1.0 / 12.0);
 
@@ -87,8 +98,18 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long* d
 
 // NF staged fields: rho, Vr, Vt, Vz, T, p, mu, kappa (kappa only with LAD; it equals mu
 // otherwise, AssignValues ModAssign.f90:22).
-template <int STAGE, bool LAD>
-__global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a) {
+// INT: the tile list holds interior tiles only -- every owned point and every window row of
+// the owned columns takes the interior stencils, and no owned point is on the axis or on a
+// boundary patch -- so the metadata word, the closures, the patches and the axis fix are
+// compiled out.
+// MODE: MODE_PLAIN (viscous), MODE_SC (viscous with the per-point coefficients of
+// ShockCapturing = 1 and zero artificial terms), MODE_LAD (with mu*, beta*, kappa*),
+// MODE_INVISCID (do_viscous_terms = 0).
+template <int STAGE, int MODE, bool INT>
+__global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void stage_kernel(StageArgs a) {
+  constexpr bool LAD = MODE == MODE_LAD;
+  constexpr bool VISC = MODE != MODE_INVISCID;
+  constexpr bool SC = MODE == MODE_SC || MODE == MODE_LAD;
   constexpr int NF = LAD ? 8 : 7;
   __shared__ double sF[NF][WR][WZ];
   __shared__ double sD[2][WR][TZ];
@@ -97,15 +118,16 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a) {
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
-  const int nblk = g.ntz * g.ntr;
-  const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  if (t >= nblk) return;
+  const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (li >= a.ntiles) return;
+  const int t = a.tiles[li];
   const int tzi = t % g.ntz, tri = t / g.ntz;
   int i0, oi_lo, oi_hi, j0, oj_lo, oj_hi;
   tile_range(tzi, a.zcuts, i0, oi_lo, oi_hi);
   tile_range(tri, a.rcuts, j0, oj_lo, oj_hi);
 
   // ---- stage the window: primitives, p = rho T, mu = T**0.666 (AssignValues)
+#pragma unroll 1
   for (int k = threadIdx.x; k < WZ * WR; k += STAGE_THREADS) {
     const int wz = k % WZ, wr = k / WZ;
     const int i = i0 - HALO + wz, j = j0 - HALO + wr;
@@ -131,12 +153,13 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a) {
   __syncthreads();
 
   // ---- dVr/dz and dVz/dz on every window row of the owned columns (for the cross terms)
+#pragma unroll 1
   for (int k = threadIdx.x; k < TZ * WR; k += STAGE_THREADS) {
     const int tz = k % TZ, wr = k / TZ;
     const int i = i0 + tz, j = j0 - HALO + wr;
     double dvr = 0.0, dvz = 0.0;
     if (i >= oi_lo && i < oi_hi && j >= 0 && j < nr) {
-      const uint32_t zr = a.meta[(long long)i + (long long)pitch * j] & 15u;
+      const uint32_t zr = INT ? ROW_INTERIOR : (a.meta[(long long)i + (long long)pitch * j] & 15u);
       const double m = a.dxidz[i];
       dvr = d1_sum(zr, &sF[1][wr][tz + HALO], 1) * m;
       dvz = d1_sum(zr, &sF[3][wr][tz + HALO], 1) * m;
@@ -152,21 +175,22 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a) {
   if (STAGE == 4) {
     const double Tst = a.qin[(long long)g.iT + (long long)pitch * g.jT + 4 * fs];
     sTst = sqrt(Tst);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (!INT && li == 0 && threadIdx.x == 0) {
       a.sc->T_stale = Tst;
       a.sc->time = a.sc->time + dt;
     }
   }
-  if (STAGE == 1 && blockIdx.x == 0 && threadIdx.x == 0) a.sc->dmax_bits = 0ull;
+  if (STAGE == 1 && !INT && li == 0 && threadIdx.x == 0) a.sc->dmax_bits = 0ull;
 
   const int tz = threadIdx.x % TZ;
+#pragma unroll 1
   for (int ty = threadIdx.x / TZ; ty < TR; ty += STAGE_THREADS / TZ) {
     const int i = i0 + tz, j = j0 + ty;
     if (i < oi_lo || i >= oi_hi || j < oj_lo || j >= oj_hi) continue;
     const int wz = tz + HALO, wr = ty + HALO;
     const long long gi = (long long)i + (long long)pitch * j;
-    const uint32_t meta = a.meta[gi];
-    const uint32_t zr = meta & 15u, rr = (meta >> 4) & 15u;
+    const uint32_t meta = INT ? ROW_INTERIOR | (ROW_INTERIOR << 4) : a.meta[gi];
+    const uint32_t zr = INT ? ROW_INTERIOR : meta & 15u, rr = INT ? ROW_INTERIOR : (meta >> 4) & 15u;
     const double mz = a.dxidz[i], mz2 = a.ddxidxidz[i];
     const double mr = a.detadr[j], mr2 = a.ddetadetadr[j];
 
@@ -181,24 +205,27 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a) {
     d.dVzdz = sD[1][wr][tz];
     d.dTdz = d1_sum(zr, &sF[4][wr][wz], 1) * mz;
     d.dpdz = d1_sum(zr, &sF[5][wr][wz], 1) * mz;
+    AJET_FENCE();
     d.drhodr = d1_sum(rr, &sF[0][wr][wz], WZ) * mr;
     d.dVrdr = d1_sum(rr, &sF[1][wr][wz], WZ) * mr;
     d.dVtdr = d1_sum(rr, &sF[2][wr][wz], WZ) * mr;
     d.dVzdr = d1_sum(rr, &sF[3][wr][wz], WZ) * mr;
     d.dTdr = d1_sum(rr, &sF[4][wr][wz], WZ) * mr;
     d.dpdr = d1_sum(rr, &sF[5][wr][wz], WZ) * mr;
+    AJET_FENCE();
 
     double rhs[5];
     const double r = a.rc[j];
-    if (j == 0) {
+    if (!INT && j == 0) {
       // the axis row is skipped by the interior loops (ModRHS.f90:59-67): rhs = -0
 #pragma unroll
       for (int k = 0; k < 5; ++k) rhs[k] = -1.0 * 0.0;
     } else {
       inviscid(q, r, d, a.pc.gam, rhs);
     }
+    AJET_FENCE();
     double mu = sF[6][wr][wz];
-    if (a.viscous) {
+    if (VISC) {
       d.dmudz = d1_sum(zr, &sF[6][wr][wz], 1) * mz;
       d.dmudr = d1_sum(rr, &sF[6][wr][wz], WZ) * mr;
       double kappa = mu;
@@ -211,30 +238,36 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a) {
         d.dkappadr = d.dmudr;
       }
       const double z2 = mz * mz, r2 = mr * mr;
+      AJET_FENCE();
       d.d2Vrdz2 = d2_sum(zr, &sF[1][wr][wz], 1) * z2 + d.dVrdz * mz2;
       d.d2Vtdz2 = d2_sum(zr, &sF[2][wr][wz], 1) * z2 + d.dVtdz * mz2;
       d.d2Vzdz2 = d2_sum(zr, &sF[3][wr][wz], 1) * z2 + d.dVzdz * mz2;
       d.d2Tdz2 = d2_sum(zr, &sF[4][wr][wz], 1) * z2 + d.dTdz * mz2;
+      AJET_FENCE();
       d.d2Vrdr2 = d2_sum(rr, &sF[1][wr][wz], WZ) * r2 + d.dVrdr * mr2;
       d.d2Vtdr2 = d2_sum(rr, &sF[2][wr][wz], WZ) * r2 + d.dVtdr * mr2;
       d.d2Vzdr2 = d2_sum(rr, &sF[3][wr][wz], WZ) * r2 + d.dVzdr * mr2;
       d.d2Tdr2 = d2_sum(rr, &sF[4][wr][wz], WZ) * r2 + d.dTdr * mr2;
+      AJET_FENCE();
       d.d2Vrdzdr = d1_sum(rr, &sD[0][wr][tz], TZ) * mr;
       d.d2Vzdzdr = d1_sum(rr, &sD[1][wr][tz], TZ) * mr;
-      if (j != 0) {
+      AJET_FENCE();
+      if (INT || j != 0) {
         double lambda = 0.0;
-        if (a.sc_flag) {
-          const double mu_phys = pow(q[4], 0.666);
+        if (SC) {
+          // beta = (3/5) T**0.666 (+ beta*), lambda = beta - (2/3) mu (ModAssign.f90:21-23,
+          // ModShockCapturing.f90:249-250); without the artificial terms mu is T**0.666 itself
+          const double mu_phys = LAD ? pow(q[4], 0.666) : mu;
           double beta = (3.0 / 5.0) * mu_phys;
           if (LAD) beta = beta + a.lad[gi + fs];
           lambda = beta - (2.0 / 3.0) * mu;
         }
-        viscous(q, r, mu, kappa, lambda, a.sc_flag != 0, d, a.pc, rhs);
+        viscous(q, r, mu, kappa, lambda, SC, d, a.pc, rhs);
       }
     }
 
     // boundary conditions (ModBC.f90:16-277), patches in bc-file order, then the sponges
-    const uint32_t sid = (meta >> 8) & 255u;
+    const uint32_t sid = INT ? 0u : (meta >> 8) & 255u;
     const double zc = a.zc[i];
     const bool top = r >= a.pc.sponge_start_top;
     const bool inflow = j >= 1 && zc <= a.pc.sponge_end_left;
@@ -287,12 +320,12 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a) {
         out[k] = (a.acc_a[gi + k * fs] + c13 * a.acc_s[gi + k * fs]) + c16 * kk;
       }
     }
-    if (STAGE == 4 && zr == ROW_BLANK) {
+    if (STAGE == 4 && !INT && zr == ROW_BLANK) {
       // blanked points skip the final update and keep the last stage value (ModRungeKutta.f90:139)
 #pragma unroll
       for (int k = 0; k < 5; ++k) out[k] = q[k];
     }
-    if (j == 0) {
+    if (!INT && j == 0) {
       // axis_conditions (ModAxis.f90:14-16) from the stage input rows 2..6
       const double a00 = c_sbp1[0][0];
       const int fl[3] = {0, 3, 4};
@@ -307,7 +340,7 @@ __global__ __launch_bounds__(STAGE_THREADS) void stage_kernel(StageArgs a) {
 #pragma unroll
     for (int k = 0; k < 5; ++k) a.qout[gi + k * fs] = out[k];
 
-    if (STAGE == 4 && zr != ROW_BLANK) {
+    if (STAGE == 4 && (INT || zr != ROW_BLANK)) {
       const double den = cfl_denominator(out[1], out[3], out[4], sTst, mz, mr, a.pc.Re, a.pc.Pr);
       if (!(den <= 1.0e300)) a.sc->nonfinite = 1;
       dmax = fmax(dmax, den);
@@ -360,28 +393,39 @@ void upload_tables(const double sbp1[9][6], const double sbp2[9][6], hipStream_t
 
 static int pad8(int n) { return (n + 7) / 8 * 8; }
 
-hipError_t launch_stage(int stage, bool lad, const StageArgs& a, hipStream_t s) {
-  const int nblk = pad8(a.g.ntz * a.g.ntr);
-  dim3 grid(nblk), block(STAGE_THREADS);
-#define AJET_LAUNCH(S, L) hipLaunchKernelGGL((stage_kernel<S, L>), grid, block, 0, s, a)
-  if (!lad) {
-    switch (stage) {
-      case 0: AJET_LAUNCH(0, false); break;
-      case 1: AJET_LAUNCH(1, false); break;
-      case 2: AJET_LAUNCH(2, false); break;
-      case 3: AJET_LAUNCH(3, false); break;
-      default: AJET_LAUNCH(4, false); break;
-    }
-  } else {
-    switch (stage) {
-      case 0: AJET_LAUNCH(0, true); break;
-      case 1: AJET_LAUNCH(1, true); break;
-      case 2: AJET_LAUNCH(2, true); break;
-      case 3: AJET_LAUNCH(3, true); break;
-      default: AJET_LAUNCH(4, true); break;
-    }
+template <int MODE, bool INT>
+static void launch_mode(int stage, dim3 grid, dim3 block, const StageArgs& a, hipStream_t s) {
+  switch (stage) {
+    case 0: hipLaunchKernelGGL((stage_kernel<0, MODE, INT>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel<1, MODE, INT>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel<2, MODE, INT>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((stage_kernel<3, MODE, INT>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((stage_kernel<4, MODE, INT>), grid, block, 0, s, a); break;
   }
-#undef AJET_LAUNCH
+}
+
+hipError_t launch_stage(int stage, int mode, bool interior, const StageArgs& a, hipStream_t s) {
+  if (a.ntiles <= 0) return hipSuccess;
+  const int nblk = pad8(a.ntiles);
+  dim3 grid(nblk), block(STAGE_THREADS);
+  switch (mode) {
+    case MODE_PLAIN:
+      interior ? launch_mode<MODE_PLAIN, true>(stage, grid, block, a, s)
+               : launch_mode<MODE_PLAIN, false>(stage, grid, block, a, s);
+      break;
+    case MODE_SC:
+      interior ? launch_mode<MODE_SC, true>(stage, grid, block, a, s)
+               : launch_mode<MODE_SC, false>(stage, grid, block, a, s);
+      break;
+    case MODE_LAD:
+      interior ? launch_mode<MODE_LAD, true>(stage, grid, block, a, s)
+               : launch_mode<MODE_LAD, false>(stage, grid, block, a, s);
+      break;
+    default:
+      interior ? launch_mode<MODE_INVISCID, true>(stage, grid, block, a, s)
+               : launch_mode<MODE_INVISCID, false>(stage, grid, block, a, s);
+      break;
+  }
   return hipGetLastError();
 }
 

@@ -263,6 +263,27 @@ static int cut_segments(int n, int T, const std::vector<int>& lo, const std::vec
   return AJET_OK;
 }
 
+// A tile is interior when no owned point is on the axis row, on a boundary patch or off the
+// interior r stencil, and every window row of its owned columns takes the interior z
+// stencil (dVr/dz and dVz/dz are staged on those rows for the cross derivatives).
+void classify_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, std::vector<int>& boundary) {
+  interior.clear();
+  boundary.clear();
+  const int ntz = (int)s.zcuts.size() - 1, ntr = (int)s.rcuts.size() - 1;
+  for (int tr = 0; tr < ntr; ++tr)
+    for (int tz = 0; tz < ntz; ++tz) {
+      const int i0 = s.zcuts[tz], i1 = s.zcuts[tz + 1], j0 = s.rcuts[tr], j1 = s.rcuts[tr + 1];
+      bool ok = j0 > 0;
+      for (int j = std::max(0, j0 - HALO); ok && j < std::min(nr, j1 + HALO); ++j)
+        for (int i = i0; ok && i < i1; ++i) {
+          const uint32_t m = s.meta[(size_t)i + (size_t)nz * j];
+          if ((m & 15u) != ROW_INTERIOR) ok = false;
+          if (j >= j0 && j < j1 && (((m >> 4) & 15u) != ROW_INTERIOR || (m >> 8) != 0u)) ok = false;
+        }
+      (ok ? interior : boundary).push_back(tr * ntz + tz);
+    }
+}
+
 int build_tiling(Setup& s, int nz, int nr, std::string& err) {
   const int BIG = 1 << 30;
   std::vector<int> zlo(nz, BIG), zhi(nz, -BIG), rlo(nr, BIG), rhi(nr, -BIG);
