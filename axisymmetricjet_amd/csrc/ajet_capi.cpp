@@ -20,6 +20,8 @@ struct ajet_ctx {
   PhysConst pc{};
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t aux = nullptr;             // boundary tiles run here beside the interior tiles
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   double *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr, *A = nullptr, *S = nullptr, *tgt = nullptr, *lad = nullptr;
   uint32_t* meta = nullptr;
   double *zc = nullptr, *rc = nullptr, *dxidz = nullptr, *ddxidxidz = nullptr, *detadr = nullptr, *ddetadetadr = nullptr;
@@ -33,6 +35,7 @@ struct ajet_ctx {
   long long logcap = 1 << 20;
   bool dt_valid = false;     // sc->dt holds a time step
   bool fused_valid = false;  // sc->dmax_bits holds the CFL reduction of the current state
+  bool pending_finalize = false;  // the next stage 1 turns that reduction into dt
   bool have_state = false;
   std::string err;
 };
@@ -101,8 +104,12 @@ int ajet_destroy(ajet_ctx* c) {
   void* bufs[] = {c->Q0, c->Q1, c->Q2, c->A, c->S, c->tgt, c->lad, c->meta, c->zc, c->rc, c->dxidz,
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
                   c->int_tiles, c->bnd_tiles, c->sc, c->dtlog};
+  if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->aux) (void)hipStreamDestroy(c->aux);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return AJET_OK;
@@ -156,6 +163,9 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   c->pc.r_top = c->s.rc[nr - 1];
 
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+  HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
   const size_t sbytes = sizeof(double) * 5 * (size_t)g.fstride;
   for (double** b : {&c->Q0, &c->Q1, &c->Q2, &c->A, &c->S, &c->tgt}) {
     HIPCHK(c, hipMalloc((void**)b, sbytes));
@@ -231,6 +241,7 @@ int ajet_set_state(ajet_ctx* c, const double* pv, const double* pv_target) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->dt_valid = false;
   c->fused_valid = false;
+  c->pending_finalize = false;
   c->have_state = true;
   return AJET_OK;
 }
@@ -262,8 +273,10 @@ static StageArgs stage_args(ajet_ctx* c) {
   a.sc = c->sc;
   a.g = c->g;
   a.pc = c->pc;
-  a.viscous = c->p.viscous;
-  a.sc_flag = c->p.shock_capturing == 1;
+  a.dtlog = c->dtlog;
+  a.logcap = c->logcap;
+  a.CFL = c->p.CFL;
+  a.dt_from_dmax = 0;
   return a;
 }
 
@@ -275,15 +288,21 @@ static int physics_mode(const ajet_params& p) {
   return (p.C_mu != 0.0 || p.C_beta != 0.0 || p.C_kappa != 0.0) ? MODE_LAD : MODE_SC;
 }
 
+// The two launches write disjoint points from the same input, so the boundary tiles run on a
+// second stream beside the interior tiles; the main stream waits for both before going on.
 static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
   const int mode = physics_mode(c->p);
+  hipError_t e;
+  if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
   a.tiles = c->bnd_tiles;
   a.ntiles = c->n_bnd;
-  hipError_t e = launch_stage(stage, mode, false, a, c->stream);
-  if (e != hipSuccess) return e;
+  if ((e = launch_stage(stage, mode, false, a, c->aux)) != hipSuccess) return e;
+  if ((e = hipEventRecord(c->ev_join, c->aux)) != hipSuccess) return e;
   a.tiles = c->int_tiles;
   a.ntiles = c->n_int;
-  return launch_stage(stage, mode, true, a, c->stream);
+  if ((e = launch_stage(stage, mode, true, a, c->stream)) != hipSuccess) return e;
+  return hipStreamWaitEvent(c->stream, c->ev_join, 0);
 }
 
 static int read_dt(ajet_ctx* c, double* dt) {
@@ -300,7 +319,15 @@ int ajet_compute_dt(ajet_ctx* c, double* dt) {
   if (!c || !c->stream) return AJET_EINVAL;
   if (!c->have_state) return fail(c, AJET_EINVAL, "no state loaded");
   HIPCHK(c, hipSetDevice(c->device));
-  hipError_t e;
+  hipError_t e = hipSuccess;
+  if (c->fused_valid && !dt) {
+    // the previous step's last stage holds the reduction; the next stage 1 finishes it
+    c->fused_valid = false;
+    c->pending_finalize = true;
+    c->dt_valid = true;
+    return AJET_OK;
+  }
+  c->pending_finalize = false;
   if (c->fused_valid) {
     e = launch_finalize(c->sc, c->p.CFL, c->g.any_blank, c->dtlog, c->logcap, c->stream);
   } else {
@@ -336,7 +363,9 @@ int ajet_rk4_stage(ajet_ctx* c, int32_t stage) {
     case 3: a.qin = c->Q2; a.qout = c->Q1; break;
     default: a.qin = c->Q1; a.qout = c->Q0; break;
   }
+  if (stage == 1 && c->pending_finalize) a.dt_from_dmax = 1;
   HIPCHK(c, launch_stages(c, stage, a));
+  if (stage == 1) c->pending_finalize = false;
   c->fused_valid = stage == 4;
   return AJET_OK;
 }

@@ -33,8 +33,10 @@ struct StageArgs {
   StepScalars* sc;
   Geometry g;
   PhysConst pc;
-  int viscous;
-  int sc_flag;             // ShockCapturing == 1 (per-point viscous coefficients)
+  double* dtlog;           // time-step log (stage 1 appends when dt_from_dmax)
+  long long logcap;
+  double CFL;
+  int dt_from_dmax;        // stage 1: dt = CFL / (max denominator of the last stage 4)
 };
 
 struct CflArgs {

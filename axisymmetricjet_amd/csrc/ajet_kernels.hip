@@ -84,6 +84,15 @@ __device__ __forceinline__ double cfl_denominator(double Vr, double Vz, double T
   return inv + 2.0 * mx * (s * s);
 }
 
+// dt = MINVAL(CFL / denominator) = CFL / MAXVAL(denominator), since a correctly rounded
+// division by a positive number is monotone; blanked points contribute 100
+// (ModInputOutput.f90:399-402).
+__device__ __forceinline__ double dt_of_dmax(unsigned long long b, double CFL, int any_blank) {
+  double dt = b == 0ull ? 100.0 : CFL / __longlong_as_double((long long)b);
+  if (any_blank && dt > 100.0) dt = 100.0;
+  return dt;
+}
+
 __device__ __forceinline__ void block_max_atomic(double v, unsigned long long* dst, double* red) {
   v = warp_max(v);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -169,7 +178,20 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
   }
   __syncthreads();
 
-  const double dt = (STAGE > 0) ? a.sc->dt : 0.0;
+  // The time step: stage 1 finishes the reduction of the previous step's last stage itself
+  // when asked to (every thread derives the same value; one writes it and logs it).  The
+  // reduction slot is cleared in stage 2, after both stage-1 launches have read it.
+  double dt = (STAGE > 0) ? a.sc->dt : 0.0;
+  if (STAGE == 1 && a.dt_from_dmax) {
+    dt = dt_of_dmax(a.sc->dmax_bits, a.CFL, g.any_blank);
+    if (!INT && li == 0 && threadIdx.x == 0) {
+      a.sc->dt = dt;
+      const long long n = a.sc->nlog;
+      if (n < a.logcap) a.dtlog[n] = dt;
+      a.sc->nlog = n + 1;
+    }
+  }
+  if (STAGE == 2 && !INT && li == 0 && threadIdx.x == 0) a.sc->dmax_bits = 0ull;
   double dmax = 0.0;
   double sTst = 0.0;
   if (STAGE == 4) {
@@ -180,7 +202,6 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
       a.sc->time = a.sc->time + dt;
     }
   }
-  if (STAGE == 1 && !INT && li == 0 && threadIdx.x == 0) a.sc->dmax_bits = 0ull;
 
   const int tz = threadIdx.x % TZ;
 #pragma unroll 1
@@ -378,9 +399,7 @@ __global__ void scalar_reset_kernel(StepScalars* sc) {
 // (ModInputOutput.f90:399-402).
 __global__ void cfl_finalize_kernel(StepScalars* sc, double* log, long long log_cap, double CFL, int any_blank) {
   if (threadIdx.x != 0) return;
-  const unsigned long long b = sc->dmax_bits;
-  double dt = b == 0ull ? 100.0 : CFL / __longlong_as_double((long long)b);
-  if (any_blank && dt > 100.0) dt = 100.0;
+  const double dt = dt_of_dmax(sc->dmax_bits, CFL, any_blank);
   sc->dt = dt;
   if (sc->nlog < log_cap) log[sc->nlog] = dt;
   sc->nlog = sc->nlog + 1;
