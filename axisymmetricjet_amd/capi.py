@@ -10,7 +10,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "libajet.so")
+LIB = os.environ.get("AJET_LIB") or os.path.join(HERE, "libajet.so")
 
 AJET_OK = 0
 ERRORS = {-1: "EINVAL", -2: "EGRID", -3: "EPATCH", -4: "ENODEV", -5: "EBLOWUP", -6: "ENONFINITE",

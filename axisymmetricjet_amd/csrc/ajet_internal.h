@@ -12,12 +12,21 @@
 namespace ajet {
 
 // Tile of the fused stage kernel: TZ x TR output points, halo 2 on every side.
-constexpr int TZ = 32;
-constexpr int TR = 16;
+#ifndef AJET_TZ
+#define AJET_TZ 32
+#endif
+#ifndef AJET_TR
+#define AJET_TR 16
+#endif
+#ifndef AJET_THREADS
+#define AJET_THREADS 256
+#endif
+constexpr int TZ = AJET_TZ;
+constexpr int TR = AJET_TR;
 constexpr int HALO = 2;
 constexpr int WZ = TZ + 2 * HALO;
 constexpr int WR = TR + 2 * HALO;
-constexpr int STAGE_THREADS = 256;
+constexpr int STAGE_THREADS = AJET_THREADS;
 
 constexpr int MAX_PATCH = 64;
 constexpr int MAX_SEQ = 64;       // distinct ordered lists of boundary patches met at one point

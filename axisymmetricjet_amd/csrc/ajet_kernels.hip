@@ -11,17 +11,17 @@ namespace ajet {
 __constant__ double c_sbp1[9][6];
 __constant__ double c_sbp2[9][6];
 
-// Interior stencils SBP1(4,0:4) and SBP2(4,0:4) (ModOperator.f90:42, 78).
-// Three 256-thread blocks fit one CU's LDS; ask the compiler for registers that let them all
-// run (at most 168 VGPRs).
+// Waves per SIMD the stage kernels are compiled for.  Two (up to 256 VGPRs) lets every
+// per-point input load issue at the top of the point's work without spilling; three blocks'
+// LDS would fit a CU, but at 168 VGPRs the point code spills (measured slower).
 #ifndef AJET_STAGE_WAVES
-#define AJET_STAGE_WAVES 3
+#define AJET_STAGE_WAVES 2
 #endif
 constexpr int STAGE_WAVES_PER_EU = AJET_STAGE_WAVES;
 
-// Scheduling fence: keeps the stencil loads of one derivative group from being hoisted
-// above the previous group, which bounds the registers live at once.
-#define AJET_FENCE() __builtin_amdgcn_sched_barrier(0)
+#define AJET_FENCE() do { } while (0)
+
+// Interior stencils SBP1(4,0:4) and SBP2(4,0:4) (ModOperator.f90:42, 78).
 
 constexpr double D1A = 1.0 / 12.0, D1B = -(2.0 / 3.0), D1D = 2.0 / 3.0, D1E = -(1.0 / 12.0);
 constexpr double D2A = -(1.0 / 12.0), D2B = 4.0 / 3.0, D2C = -(5.0 / 2.0), D2D = 4.0 / 3.0, D2E = -(1.0 / 12.0);
@@ -135,26 +135,45 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
   tile_range(tzi, a.zcuts, i0, oi_lo, oi_hi);
   tile_range(tri, a.rcuts, j0, oj_lo, oj_hi);
 
-  // ---- stage the window: primitives, p = rho T, mu = T**0.666 (AssignValues)
-#pragma unroll 1
-  for (int k = threadIdx.x; k < WZ * WR; k += STAGE_THREADS) {
+  // ---- stage the window: primitives, p = rho T, mu = T**0.666 (AssignValues).  Every load
+  // of the thread's window points is issued before any of them is used, so the block waits
+  // for memory once.
+  constexpr int NWIN = (WZ * WR + STAGE_THREADS - 1) / STAGE_THREADS;
+  double win[NWIN][5];
+  bool inside[NWIN];
+#pragma unroll
+  for (int u = 0; u < NWIN; ++u) {
+    const int k = threadIdx.x + u * STAGE_THREADS;
     const int wz = k % WZ, wr = k / WZ;
     const int i = i0 - HALO + wz, j = j0 - HALO + wr;
-    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, p = 0.0, mu = 0.0, kap = 0.0;
-    if (i >= 0 && i < nz && j >= 0 && j < nr) {
-      const long long gi = (long long)i + (long long)pitch * j;
+    inside[u] = k < WZ * WR && i >= 0 && i < nz && j >= 0 && j < nr;
+    // points outside the grid load point 0 (a valid address) and are zeroed after the load,
+    // so every load issues without a branch
+    const long long gi = inside[u] ? (long long)i + (long long)pitch * j : 0;
 #pragma unroll
-      for (int f = 0; f < 5; ++f) v[f] = a.qin[gi + f * fs];
-      p = v[0] * v[4];
-      mu = pow(v[4], 0.666);
+    for (int f = 0; f < 5; ++f) {
+      const double x = a.qin[gi + f * fs];
+      win[u][f] = inside[u] ? x : 0.0;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NWIN; ++u) {
+    const int k = threadIdx.x + u * STAGE_THREADS;
+    if (k >= WZ * WR) break;
+    const int wz = k % WZ, wr = k / WZ;
+    double p = 0.0, mu = 0.0, kap = 0.0;
+    if (inside[u]) {
+      p = win[u][0] * win[u][4];
+      mu = pow(win[u][4], 0.666);
       kap = mu;
       if (LAD) {
+        const long long gi = (long long)(i0 - HALO + wz) + (long long)pitch * (j0 - HALO + wr);
         mu = mu + a.lad[gi];
         kap = kap + a.lad[gi + 2 * fs];
       }
     }
 #pragma unroll
-    for (int f = 0; f < 5; ++f) sF[f][wr][wz] = v[f];
+    for (int f = 0; f < 5; ++f) sF[f][wr][wz] = win[u][f];
     sF[5][wr][wz] = p;
     sF[6][wr][wz] = mu;
     if (LAD) sF[NF - 1][wr][wz] = kap;
@@ -204,7 +223,10 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
   }
 
   const int tz = threadIdx.x % TZ;
-#pragma unroll 1
+#ifndef AJET_POINT_UNROLL
+#define AJET_POINT_UNROLL 1
+#endif
+#pragma unroll AJET_POINT_UNROLL
   for (int ty = threadIdx.x / TZ; ty < TR; ty += STAGE_THREADS / TZ) {
     const int i = i0 + tz, j = j0 + ty;
     if (i < oi_lo || i >= oi_hi || j < oj_lo || j >= oj_hi) continue;
@@ -214,6 +236,30 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
     const uint32_t zr = INT ? ROW_INTERIOR : meta & 15u, rr = INT ? ROW_INTERIOR : (meta >> 4) & 15u;
     const double mz = a.dxidz[i], mz2 = a.ddxidxidz[i];
     const double mr = a.detadr[j], mr2 = a.ddetadetadr[j];
+    const double r = a.rc[j];
+    const double zc = a.zc[i];
+    // pointwise inputs of the stage update, loaded before the stencil work so that their
+    // latency hides behind it: stage 2, 3: q^n (and S); stage 4: A and S
+    double pin0[5], pin1[5];
+    if (STAGE >= 2) {
+      const double* __restrict__ src0 = STAGE == 4 ? a.acc_a : a.qn;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) pin0[k] = src0[gi + k * fs];
+    }
+    if (STAGE >= 3) {
+      const double* __restrict__ src1 = a.acc_s;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) pin1[k] = src1[gi + k * fs];
+    }
+    // the target where a boundary patch or a sponge needs it
+    const uint32_t sid = INT ? 0u : (meta >> 8) & 255u;
+    const bool top = r >= a.pc.sponge_start_top;
+    const bool inflow = j >= 1 && zc <= a.pc.sponge_end_left;
+    double qt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (sid != 0 || top || inflow) {
+#pragma unroll
+      for (int f = 0; f < 5; ++f) qt[f] = a.tgt[gi + f * fs];
+    }
 
     double q[5];
 #pragma unroll
@@ -236,7 +282,6 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
     AJET_FENCE();
 
     double rhs[5];
-    const double r = a.rc[j];
     if (!INT && j == 0) {
       // the axis row is skipped by the interior loops (ModRHS.f90:59-67): rhs = -0
 #pragma unroll
@@ -288,14 +333,7 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
     }
 
     // boundary conditions (ModBC.f90:16-277), patches in bc-file order, then the sponges
-    const uint32_t sid = INT ? 0u : (meta >> 8) & 255u;
-    const double zc = a.zc[i];
-    const bool top = r >= a.pc.sponge_start_top;
-    const bool inflow = j >= 1 && zc <= a.pc.sponge_end_left;
     if (sid != 0 || top || inflow) {
-      double qt[5];
-#pragma unroll
-      for (int f = 0; f < 5; ++f) qt[f] = a.tgt[gi + f * fs];
       if (sid != 0) {
         for (int s = 0; s < SEQ_LEN; ++s) {
           const int pidx = a.seq[sid * SEQ_LEN + s];
@@ -322,24 +360,33 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
       continue;
     }
 
-    // ---- stage update (ModRungeKutta.f90:32-146)
-    double out[5];
+    // ---- stage update (ModRungeKutta.f90:32-146), from the inputs loaded at the top
+    double out[5], acc[5];
     const double c16 = 1.0 / 6.0, c13 = 1.0 / 3.0;
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       const double kk = dt * rhs[k];
       if (STAGE == 1) {
         out[k] = q[k] + 0.5 * kk;
-        a.acc_a[gi + k * fs] = q[k] + c16 * kk;
+        acc[k] = q[k] + c16 * kk;
       } else if (STAGE == 2) {
-        out[k] = a.qn[gi + k * fs] + 0.5 * kk;
-        a.acc_s[gi + k * fs] = kk;
+        out[k] = pin0[k] + 0.5 * kk;
+        acc[k] = kk;
       } else if (STAGE == 3) {
-        out[k] = a.qn[gi + k * fs] + kk;
-        a.acc_s[gi + k * fs] = a.acc_s[gi + k * fs] + kk;
+        out[k] = pin0[k] + kk;
+        acc[k] = pin1[k] + kk;
       } else {
-        out[k] = (a.acc_a[gi + k * fs] + c13 * a.acc_s[gi + k * fs]) + c16 * kk;
+        out[k] = (pin0[k] + c13 * pin1[k]) + c16 * kk;
       }
+    }
+    if (STAGE == 1) {
+      double* __restrict__ accA = a.acc_a;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) accA[gi + k * fs] = acc[k];
+    } else if (STAGE == 2 || STAGE == 3) {
+      double* __restrict__ accS = a.acc_s;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) accS[gi + k * fs] = acc[k];
     }
     if (STAGE == 4 && !INT && zr == ROW_BLANK) {
       // blanked points skip the final update and keep the last stage value (ModRungeKutta.f90:139)
