@@ -1,0 +1,97 @@
+! ISO_C_BINDING mirror of include/ajet_capi.h: the derived types are bind(C) copies of the C
+! structs and every entry point is an interface to the gfx950 library libajet.so.
+MODULE ajet_capi
+  USE, INTRINSIC :: ISO_C_BINDING
+  IMPLICIT NONE
+
+  INTEGER(C_INT), PARAMETER :: AJET_OK = 0, AJET_EBLOWUP = -5, AJET_ENONFINITE = -6
+
+  ! ajet_params (solver constants; ModuleVariables.f90:12-13, 66, 71, 73, 152, 164, 177)
+  TYPE, BIND(C) :: ajet_params_t
+    INTEGER(C_INT32_T) :: nz, nr, viscous, shock_capturing, forcing, reserved
+    REAL(C_DOUBLE) :: gam, Re, Pr, CFL, C_mu, C_beta, C_kappa
+    REAL(C_DOUBLE) :: sponge_start_top, sponge_A_top, sponge_end_left, sponge_A_left
+  END TYPE ajet_params_t
+
+  ! one bc-file line: grid, bc, bc_dir, is, ie, js, je (ModPatch.f90:49)
+  TYPE, BIND(C) :: ajet_bc_t
+    INTEGER(C_INT64_T) :: grid, bc, bc_dir, i_start, i_end, j_start, j_end
+  END TYPE ajet_bc_t
+
+  INTERFACE
+    SUBROUTINE ajet_params_default(p, nz, nr) BIND(C, name='ajet_params_default')
+      IMPORT :: ajet_params_t, C_INT32_T
+      TYPE(ajet_params_t), INTENT(OUT) :: p
+      INTEGER(C_INT32_T), VALUE :: nz, nr
+    END SUBROUTINE ajet_params_default
+
+    INTEGER(C_INT) FUNCTION ajet_create(p, z, r, iblank, lines, nlines, ctx) BIND(C, name='ajet_create')
+      IMPORT :: ajet_params_t, ajet_bc_t, C_INT, C_DOUBLE, C_INT32_T, C_PTR
+      TYPE(ajet_params_t), INTENT(IN) :: p
+      REAL(C_DOUBLE), INTENT(IN) :: z(*), r(*)
+      INTEGER(C_INT32_T), INTENT(IN) :: iblank(*)
+      TYPE(ajet_bc_t), INTENT(IN) :: lines(*)
+      INTEGER(C_INT32_T), VALUE :: nlines
+      TYPE(C_PTR), INTENT(OUT) :: ctx
+    END FUNCTION ajet_create
+
+    INTEGER(C_INT) FUNCTION ajet_destroy(ctx) BIND(C, name='ajet_destroy')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION ajet_destroy
+
+    TYPE(C_PTR) FUNCTION ajet_last_error_c(ctx) BIND(C, name='ajet_last_error')
+      IMPORT :: C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION ajet_last_error_c
+
+    INTEGER(C_INT) FUNCTION ajet_set_state(ctx, pv, pv_target) BIND(C, name='ajet_set_state')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(IN) :: pv(*), pv_target(*)
+    END FUNCTION ajet_set_state
+
+    INTEGER(C_INT) FUNCTION ajet_get_state(ctx, pv) BIND(C, name='ajet_get_state')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE
+      TYPE(C_PTR), VALUE :: ctx
+      REAL(C_DOUBLE), INTENT(OUT) :: pv(*)
+    END FUNCTION ajet_get_state
+
+    ! dt: C_NULL_PTR keeps the time step on the device (no synchronisation)
+    INTEGER(C_INT) FUNCTION ajet_compute_dt(ctx, dt) BIND(C, name='ajet_compute_dt')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx, dt
+    END FUNCTION ajet_compute_dt
+
+    INTEGER(C_INT) FUNCTION ajet_rk4_step(ctx) BIND(C, name='ajet_rk4_step')
+      IMPORT :: C_INT, C_PTR
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION ajet_rk4_step
+
+    INTEGER(C_INT) FUNCTION ajet_get_dt_log(ctx, dts, n, time) BIND(C, name='ajet_get_dt_log')
+      IMPORT :: C_INT, C_PTR, C_INT32_T
+      TYPE(C_PTR), VALUE :: ctx, dts, time
+      INTEGER(C_INT32_T), VALUE :: n
+    END FUNCTION ajet_get_dt_log
+  END INTERFACE
+
+CONTAINS
+
+  ! ajet_last_error as a Fortran string
+  FUNCTION ajet_last_error(ctx) RESULT(msg)
+    TYPE(C_PTR), INTENT(IN) :: ctx
+    CHARACTER(LEN=256) :: msg
+    CHARACTER(KIND=C_CHAR), POINTER :: s(:)
+    INTEGER :: k
+    TYPE(C_PTR) :: p
+    msg = ' '
+    p = ajet_last_error_c(ctx)
+    IF (.NOT. C_ASSOCIATED(p)) RETURN
+    CALL C_F_POINTER(p, s, [256])
+    DO k = 1, 256
+      IF (s(k) == C_NULL_CHAR) EXIT
+      msg(k:k) = s(k)
+    END DO
+  END FUNCTION ajet_last_error
+
+END MODULE ajet_capi

@@ -1,0 +1,43 @@
+"""GPU test of the Fortran host: PROGRAM ModSolver (ajet_solver) reads the PLOT3D grid, initial
+condition, target and bc file, runs {ComputeTimeStep; RK4} on the GPU through the C ABI and
+writes RocFlo-CM.########.q files; the result must match the reference's own trajectory."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from axisymmetricjet_amd import jetcase as jc
+import cases as gc
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SOLVER = os.path.join(ROOT, "axisymmetricjet_amd", "ajet_solver")
+
+
+def test_fortran_solver_matches_reference_trajectory(tmp_path):
+    case, g, shock, nsteps, extra = gc.load("jet88")
+    paths = jc.write_case(str(tmp_path / "c_"), case)
+    nml = tmp_path / "ajet.nml"
+    nml.write_text(
+        "&ajet_input\n"
+        f" GRID_FILENAME='{paths['grid']}', INITIAL_CONDITION='{paths['ic']}',\n"
+        f" TARGET_FILE='{paths['target']}', bc_file='{paths['bc']}', out_prefix='{tmp_path}/',\n"
+        f" Re={case.re!r}, Pr={case.pr!r}, CFL=0.5, no_iterations={nsteps}, noutput=10\n/\n")
+    print("running ajet_solver on jet88", flush=True)
+    out = subprocess.run([SOLVER, str(nml)], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = re.findall(r"Iteration=(\d+)\s+dt=\s*(\S+)\s+time=\s*(\S+)", out.stdout)
+    assert [int(k) for k, _, _ in lines] == list(range(1, nsteps + 1))
+    dts = np.array([float(d) for _, d, _ in lines])
+    np.testing.assert_allclose(dts, g["dts"], rtol=1e-8)          # E17.9 print precision
+    for step in (0, 10, 20):
+        assert (tmp_path / f"RocFlo-CM.{step:08d}.q").exists()
+    q, tau = jc.read_soln(str(tmp_path / f"RocFlo-CM.{nsteps:08d}.q"))
+    assert tau[0] == float(nsteps)
+    assert abs(tau[3] - float(g["time"])) <= 1e-12 * float(g["time"])
+    err = np.abs(q - g["state"]).max()
+    print(f"fortran host: state max-norm err {err:.3e}", flush=True)
+    assert err <= 1e-9

@@ -1,0 +1,85 @@
+"""CPU tests of the boundary: the C ABI exports, the patch builder behind it, and the Fortran
+host's PLOT3D reader/writer (byte-identical to the Write_Soln layout)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from axisymmetricjet_amd import capi
+from axisymmetricjet_amd import jetcase as jc
+from oracle import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HOST_BUILD = os.path.join(ROOT, "axisymmetricjet_amd", "_build", "host")
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "ajet_capi.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*|void\*)\s+\**(ajet_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    names = header_functions()
+    assert len(names) >= 15
+    L = C.CDLL(capi.LIB)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(capi.EXPORTS)
+
+
+def test_params_default_are_reference_constants():
+    p = capi.default_params(100, 50)
+    assert (p.nz, p.nr, p.viscous, p.shock_capturing, p.forcing) == (100, 50, 1, 0, 0)
+    assert (p.gam, p.Re, p.Pr, p.CFL) == (1.4, 1000.0, 0.72, 0.5)
+    assert (p.C_mu, p.C_beta, p.C_kappa) == (0.0, 0.0, 0.0)
+    assert (p.sponge_start_top, p.sponge_A_top, p.sponge_end_left, p.sponge_A_left) == (12.5, 0.5, 6.0, 5.0)
+
+
+@pytest.mark.parametrize("case", [
+    jc.make_jet(88, 32, seed=1),
+    jc.make_nozzle(96, 48, i_lip=24, j_in=16, j_out=22),
+    jc.make_jet(440, 64, seed=1, bc_lines=jc.NOZZLE_BC_4400x640[:0] + jc.DEFAULT_BC),
+])
+def test_patches_match_oracle(case):
+    got = capi.build_patches(case.nz, case.nr, case.bc_lines)
+    assert got == po.Oracle(case).patches()
+
+
+def test_nozzle_bc_file_patches_at_full_size():
+    """The one bc file the reference ships, on its 4400 x 640 grid."""
+    got = capi.build_patches(4400, 640, jc.NOZZLE_BC_4400x640)
+    assert got[0] == (1, 4, 1, 150, 91, 1)
+    assert got[1] == (1, 597, 147, 150, 94, -2)
+    assert got[2] == (597, 600, 150, 169, 94, 1)
+    assert got[3] == (1, 597, 169, 172, 94, 2)
+    assert got[5] == (1, 4400, 637, 640, 92, -2)
+    assert got[6] == (4397, 4400, 1, 640, 92, -1)
+    assert got[7] == (1, 4400, 1, 4, 90, 2)
+
+
+def ensure_host_built():
+    exe = os.path.join(HOST_BUILD, "p3d_roundtrip")
+    if not os.path.exists(exe):
+        subprocess.check_call(["bash", os.path.join(ROOT, "axisymmetricjet_amd", "host", "build_host.sh")])
+    return exe
+
+
+def test_fortran_plot3d_roundtrip_is_byte_identical(tmp_path):
+    exe = ensure_host_built()
+    case = jc.make_nozzle(96, 48, i_lip=24, j_in=16, j_out=22)
+    grid = str(tmp_path / "g.xyz")
+    sin = str(tmp_path / "in.q")
+    sout = str(tmp_path / "out.q")
+    jc.write_grid(grid, case.z, case.r, case.iblank)
+    jc.write_soln(sin, case.q0, tau=(12.0, 0.1, 0.2, 3.5))
+    out = subprocess.check_output([exe, grid, sin, sout], text=True).split()
+    nz, nr, ibsum = int(out[0]), int(out[1]), int(out[2])
+    assert (nz, nr, ibsum) == (96, 48, int(case.iblank.sum()))
+    np.testing.assert_allclose(float(out[5]), case.q0.sum(), rtol=1e-14)
+    assert open(sin, "rb").read() == open(sout, "rb").read()
+    q, tau = jc.read_soln(sout)
+    assert tau == (12.0, 0.1, 0.2, 3.5)
+    assert np.array_equal(q, case.q0)
