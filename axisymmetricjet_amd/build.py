@@ -12,8 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libajet.so")
-SOURCES = ["ajet_capi.cpp", "ajet_setup.cpp", "ajet_kernels.hip"]
-HEADERS = ["ajet_internal.h", "ajet_kernels.h", "ajet_physics.h"]
+SOURCES = ["ajet_kernels.hip", "ajet_lad.hip", "ajet_capi.cpp", "ajet_setup.cpp"]
+HEADERS = ["ajet_internal.h", "ajet_kernels.h", "ajet_physics.h", "ajet_stencil.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: every multiply and add rounds on its own, as in the reference's
@@ -24,7 +24,8 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
+    """Compile every source (in parallel) and link libajet.so; raises on any compiler error."""
+    objs, procs = [], []
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
     for src in SOURCES:
         obj = os.path.join(HERE, "_build", src + ".o")
@@ -33,8 +34,11 @@ def build(verbose: bool = False) -> str:
             cmd[1:1] = ["-x", "hip"]
         if verbose:
             print(" ".join(shlex.quote(x) for x in cmd), flush=True)
-        subprocess.check_call(cmd)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, "hipcc " + " ".join(failed))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.check_call(cmd)

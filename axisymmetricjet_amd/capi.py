@@ -189,5 +189,11 @@ class Context:
         self._check(lib().ajet_rhs(self.h, F(q), out), "ajet_rhs")
         return out
 
+    def lad_coeffs(self, q):
+        """mu*, beta*, kappa* of ComputeArtTransCoeffs for the state q, each (nz, nr)."""
+        outs = [np.zeros((self.nz, self.nr), order="F") for _ in range(3)]
+        self._check(lib().ajet_lad_coeffs(self.h, F(q), *outs), "ajet_lad_coeffs")
+        return outs
+
     def stream(self):
         return lib().ajet_stream(self.h)

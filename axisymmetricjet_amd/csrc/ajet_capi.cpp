@@ -22,7 +22,9 @@ struct ajet_ctx {
   hipStream_t stream = nullptr;
   hipStream_t aux = nullptr;             // boundary tiles run here beside the interior tiles
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  double *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr, *A = nullptr, *S = nullptr, *tgt = nullptr, *lad = nullptr;
+  double *Q0 = nullptr, *Q1 = nullptr, *Q2 = nullptr, *A = nullptr, *S = nullptr, *tgt = nullptr;
+  double *lad = nullptr, *ladA = nullptr, *ladB = nullptr;  // mu*, beta*, kappa* and two 3-field scratch sets
+  double *dzl = nullptr, *drl = nullptr, *ramp = nullptr;   // LAD grid spacings and ramp (1-D)
   uint32_t* meta = nullptr;
   double *zc = nullptr, *rc = nullptr, *dxidz = nullptr, *ddxidxidz = nullptr, *detadr = nullptr, *ddetadetadr = nullptr;
   PatchDev* patches = nullptr;
@@ -30,6 +32,8 @@ struct ajet_ctx {
   int *zcuts = nullptr, *rcuts = nullptr;
   int *int_tiles = nullptr, *bnd_tiles = nullptr;  // interior and boundary tile lists
   int n_int = 0, n_bnd = 0;
+  int *lad_int = nullptr, *lad_bnd = nullptr;      // the same split for the LAD kernels
+  int n_lad_int = 0, n_lad_bnd = 0;
   StepScalars* sc = nullptr;
   double* dtlog = nullptr;
   long long logcap = 1 << 20;
@@ -101,9 +105,10 @@ int ajet_destroy(ajet_ctx* c) {
   if (!c) return AJET_EINVAL;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* bufs[] = {c->Q0, c->Q1, c->Q2, c->A, c->S, c->tgt, c->lad, c->meta, c->zc, c->rc, c->dxidz,
+  void* bufs[] = {c->Q0, c->Q1, c->Q2, c->A, c->S, c->tgt, c->lad, c->ladA, c->ladB, c->dzl, c->drl, c->ramp,
+                  c->meta, c->zc, c->rc, c->dxidz,
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
-                  c->int_tiles, c->bnd_tiles, c->sc, c->dtlog};
+                  c->int_tiles, c->bnd_tiles, c->lad_int, c->lad_bnd, c->sc, c->dtlog};
   if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -125,8 +130,6 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   int rc = build_setup(*p, z, r, iblank, lines, nlines, c->s, c->err);
   if (rc == AJET_OK) rc = build_tiling(c->s, p->nz, p->nr, c->err);
   if (rc == AJET_OK && p->forcing != 0) rc = fail(c, AJET_EUNSUPPORTED, "inflow forcing is not available");
-  if (rc == AJET_OK && p->shock_capturing == 1 && (p->C_mu != 0.0 || p->C_beta != 0.0 || p->C_kappa != 0.0))
-    rc = fail(c, AJET_EUNSUPPORTED, "artificial diffusivity with non-zero constants is not available yet");
   int ndev = 0;
   if (rc == AJET_OK && (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1))
     rc = fail(c, AJET_ENODEV, "no HIP device");
@@ -172,17 +175,22 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
     HIPCHK(c, hipMemsetAsync(*b, 0, sbytes, c->stream));
   }
   if (p->shock_capturing == 1) {
-    HIPCHK(c, hipMalloc((void**)&c->lad, sizeof(double) * 3 * (size_t)g.fstride));
-    HIPCHK(c, hipMemsetAsync(c->lad, 0, sizeof(double) * 3 * (size_t)g.fstride, c->stream));
+    for (double** b : {&c->lad, &c->ladA, &c->ladB}) {
+      HIPCHK(c, hipMalloc((void**)b, sizeof(double) * 3 * (size_t)g.fstride));
+      HIPCHK(c, hipMemsetAsync(*b, 0, sizeof(double) * 3 * (size_t)g.fstride, c->stream));
+    }
   }
   std::vector<uint32_t> meta((size_t)g.fstride, ROW_BLANK | (ROW_BLANK << 4));
   for (int j = 0; j < nr; ++j)
     for (int i = 0; i < nz; ++i) meta[(size_t)i + (size_t)g.pitch * j] = c->s.meta[(size_t)i + (size_t)nz * j];
   HIPCHK(c, hipMalloc((void**)&c->meta, sizeof(uint32_t) * meta.size()));
   HIPCHK(c, hipMemcpyAsync(c->meta, meta.data(), sizeof(uint32_t) * meta.size(), hipMemcpyHostToDevice, c->stream));
+  std::vector<double> dzl, drl, ramp;
+  lad_tables(c->s, nz, nr, dzl, drl, ramp);
   struct V { double** d; const std::vector<double>* h; } vs[] = {
       {&c->zc, &c->s.zc}, {&c->rc, &c->s.rc}, {&c->dxidz, &c->s.dxidz}, {&c->ddxidxidz, &c->s.ddxidxidz},
-      {&c->detadr, &c->s.detadr}, {&c->ddetadetadr, &c->s.ddetadetadr}};
+      {&c->detadr, &c->s.detadr}, {&c->ddetadetadr, &c->s.ddetadetadr}, {&c->dzl, &dzl}, {&c->drl, &drl},
+      {&c->ramp, &ramp}};
   for (auto& v : vs) {
     HIPCHK(c, hipMalloc((void**)v.d, sizeof(double) * v.h->size()));
     HIPCHK(c, hipMemcpyAsync(*v.d, v.h->data(), sizeof(double) * v.h->size(), hipMemcpyHostToDevice, c->stream));
@@ -214,13 +222,23 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
       HIPCHK(c, hipMemcpy(c->int_tiles, it.data(), sizeof(int) * it.size(), hipMemcpyHostToDevice));
     if (!bt.empty())
       HIPCHK(c, hipMemcpy(c->bnd_tiles, bt.data(), sizeof(int) * bt.size(), hipMemcpyHostToDevice));
+    classify_lad_tiles(c->s, nz, nr, it, bt);
+    c->n_lad_int = (int)it.size();
+    c->n_lad_bnd = (int)bt.size();
+    HIPCHK(c, hipMalloc((void**)&c->lad_int, sizeof(int) * (it.size() + 1)));
+    HIPCHK(c, hipMalloc((void**)&c->lad_bnd, sizeof(int) * (bt.size() + 1)));
+    if (!it.empty())
+      HIPCHK(c, hipMemcpy(c->lad_int, it.data(), sizeof(int) * it.size(), hipMemcpyHostToDevice));
+    if (!bt.empty())
+      HIPCHK(c, hipMemcpy(c->lad_bnd, bt.data(), sizeof(int) * bt.size(), hipMemcpyHostToDevice));
   }
   HIPCHK(c, hipMalloc((void**)&c->sc, sizeof(StepScalars)));
   HIPCHK(c, hipMemsetAsync(c->sc, 0, sizeof(StepScalars), c->stream));
   HIPCHK(c, hipMalloc((void**)&c->dtlog, sizeof(double) * (size_t)c->logcap));
   double A[9][6], B[9][6], F[9][11];
   sbp_tables(A, B, F);
-  upload_tables(A, B, c->stream);
+  HIPCHK(c, upload_tables(A, B, F, c->stream));
+  HIPCHK(c, upload_lad_tables(A, B, F, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   *out = c;
   return AJET_OK;
@@ -288,11 +306,42 @@ static int physics_mode(const ajet_params& p) {
   return (p.C_mu != 0.0 || p.C_beta != 0.0 || p.C_kappa != 0.0) ? MODE_LAD : MODE_SC;
 }
 
-// The two launches write disjoint points from the same input, so the boundary tiles run on a
+static LadArgs lad_args(ajet_ctx* c, const double* q) {
+  LadArgs l{};
+  l.q = q;
+  l.meta = c->meta;
+  l.dxidz = c->dxidz;
+  l.detadr = c->detadr;
+  l.rc = c->rc;
+  l.dzl = c->dzl;
+  l.drl = c->drl;
+  l.ramp = c->ramp;
+  l.A = c->ladA;
+  l.B = c->ladB;
+  l.out = c->lad;
+  l.zcuts = c->zcuts;
+  l.rcuts = c->rcuts;
+  l.int_tiles = c->lad_int;
+  l.bnd_tiles = c->lad_bnd;
+  l.n_int = c->n_lad_int;
+  l.n_bnd = c->n_lad_bnd;
+  l.g = c->g;
+  l.gam = c->p.gam;
+  // ModShockCapturing.f90:66, 72, 79: C_mu Re, C_beta Re, C_kappa Re Pr
+  l.cmu = c->p.C_mu * c->p.Re;
+  l.cbeta = c->p.C_beta * c->p.Re;
+  l.ckappa = c->p.C_kappa * c->p.Re * c->p.Pr;
+  return l;
+}
+
+// The artificial coefficients of the stage input come first (ComputeRHS calls
+// ComputeArtTransCoeffs after the first derivatives, ModRHS.f90:46-47).  The two stage
+// launches then write disjoint points from the same input, so the boundary tiles run on a
 // second stream beside the interior tiles; the main stream waits for both before going on.
 static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
   const int mode = physics_mode(c->p);
   hipError_t e;
+  if (mode == MODE_LAD && (e = launch_lad(lad_args(c, a.qin), c->stream)) != hipSuccess) return e;
   if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
   a.tiles = c->bnd_tiles;
@@ -423,8 +472,18 @@ int ajet_rhs(ajet_ctx* c, const double* q, double* rhs) {
 }
 
 int ajet_lad_coeffs(ajet_ctx* c, const double* q, double* mustar, double* betastar, double* kappastar) {
-  (void)q; (void)mustar; (void)betastar; (void)kappastar;
-  return fail(c, AJET_EUNSUPPORTED, "artificial diffusivity kernels are not available yet");
+  if (!c || !c->stream || !q || !mustar || !betastar || !kappastar) return AJET_EINVAL;
+  if (c->p.shock_capturing != 1) return fail(c, AJET_EINVAL, "shock capturing is off in this context");
+  HIPCHK(c, hipSetDevice(c->device));
+  copy_state_in(c, c->Q2, q);
+  HIPCHK(c, launch_lad(lad_args(c, c->Q2), c->stream));
+  double* outs[3] = {mustar, betastar, kappastar};
+  for (int f = 0; f < 3; ++f)
+    HIPCHK(c, hipMemcpy2DAsync(outs[f], sizeof(double) * c->g.nz, c->lad + (size_t)f * c->g.fstride,
+                               sizeof(double) * c->g.pitch, sizeof(double) * c->g.nz, (size_t)c->g.nr,
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AJET_OK;
 }
 
 }  // extern "C"

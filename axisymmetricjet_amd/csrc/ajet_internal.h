@@ -84,5 +84,8 @@ int build_setup(const ajet_params& p, const double* z, const double* r, const in
                 const ajet_bc* lines, int nlines, Setup& s, std::string& err);
 int build_tiling(Setup& s, int nz, int nr, std::string& err);
 void classify_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, std::vector<int>& boundary);
+void classify_lad_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, std::vector<int>& boundary);
+void lad_tables(const Setup& s, int nz, int nr, std::vector<double>& dz, std::vector<double>& dr,
+                std::vector<double>& ramp);
 
 }  // namespace ajet

@@ -49,7 +49,35 @@ struct CflArgs {
   double Re, Pr;
 };
 
-void upload_tables(const double sbp1[9][6], const double sbp2[9][6], hipStream_t s);
+// Artificial transport coefficients (ComputeArtTransCoeffs, ModShockCapturing.f90:5-106) of
+// the state q, in four launches: strain rate / dilatation / switch, the filtered-quantity
+// sources, the in-place z filter, the r filter with the C Re scaling and the ramp.
+struct LadArgs {
+  const double* q;         // state whose coefficients are wanted (5 fields)
+  const uint32_t* meta;
+  const double* dxidz;     // (nz)
+  const double* detadr;    // (nr)
+  const double* rc;        // r(j)
+  const double* dzl;       // grid spacing of ComputeTobeFilteredQuantities along z (nz)
+  const double* drl;       // ... along r (nr)
+  const double* ramp;      // (1 + tanh(5 (z - 10))) / 2 (nz)
+  double* A;               // 3 fields: S, dil, f_sw; later the z-filtered quantities
+  double* B;               // 3 fields: the quantities before filtering
+  double* out;             // 3 fields: mu*, beta*, kappa*
+  const int* zcuts;
+  const int* rcuts;
+  const int* int_tiles;    // tiles whose points and stencil rows are all interior
+  const int* bnd_tiles;    // the others
+  int n_int, n_bnd;
+  Geometry g;
+  double gam;
+  double cmu, cbeta, ckappa;  // C_mu Re, C_beta Re, C_kappa Re Pr
+};
+
+hipError_t upload_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
+hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
+hipError_t launch_lad(const LadArgs& a, hipStream_t s);
+
 // Physics variants of the stage kernel.
 enum { MODE_PLAIN = 0, MODE_SC = 1, MODE_LAD = 2, MODE_INVISCID = 3 };
 

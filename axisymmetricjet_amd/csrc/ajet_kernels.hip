@@ -5,11 +5,9 @@
 
 #include "ajet_kernels.h"
 #include "ajet_physics.h"
+#include "ajet_stencil.h"
 
 namespace ajet {
-
-__constant__ double c_sbp1[9][6];
-__constant__ double c_sbp2[9][6];
 
 // Waves per SIMD the stage kernels are compiled for.  Two (up to 256 VGPRs) lets every
 // per-point input load issue at the top of the point's work without spilling; three blocks'
@@ -21,46 +19,12 @@ constexpr int STAGE_WAVES_PER_EU = AJET_STAGE_WAVES;
 
 #define AJET_FENCE() do { } while (0)
 
-// Interior stencils SBP1(4,0:4) and SBP2(4,0:4) (ModOperator.f90:42, 78).
-
-constexpr double D1A = 1.0 / 12.0, D1B = -(2.0 / 3.0), D1D = 2.0 / 3.0, D1E = -(1.0 / 12.0);
-constexpr double D2A = -(1.0 / 12.0), D2B = 4.0 / 3.0, D2C = -(5.0 / 2.0), D2D = 4.0 / 3.0, D2E = -(1.0 / 12.0);
-
-// SBP first-derivative stencil sum (before the metric) at a point whose row code is `row`;
-// `c` points at the point itself inside a strided line, stride s.  The interior sum skips
-// the zero centre weight, which adds an exact zero in the reference.
-__device__ __forceinline__ double d1_sum(uint32_t row, const double* c, int s) {
-  if (row == ROW_INTERIOR) return ((D1A * c[-2 * s] + D1B * c[-s]) + D1D * c[s]) + D1E * c[2 * s];
-  if (row == ROW_BLANK) return 0.0;
-  const int base = row < 4 ? -(int)row : -(int)row + 3;
-  const double* v = c + base * s;
-  double acc = 0.0;
-#pragma unroll
-  for (int m = 0; m < 6; ++m) acc = acc + c_sbp1[row][m] * v[m * s];
-  return acc;
-}
-
-__device__ __forceinline__ double d2_sum(uint32_t row, const double* c, int s) {
-  if (row == ROW_INTERIOR) return (((D2A * c[-2 * s] + D2B * c[-s]) + D2C * c[0]) + D2D * c[s]) + D2E * c[2 * s];
-  if (row == ROW_BLANK) return 0.0;
-  const int base = row < 4 ? -(int)row : -(int)row + 3;
-  const double* v = c + base * s;
-  double acc = 0.0;
-#pragma unroll
-  for (int m = 0; m < 6; ++m) acc = acc + c_sbp2[row][m] * v[m * s];
-  return acc;
-}
-
 // Tile t owns [cuts[t], cuts[t+1]) (at most TZ or TR points); its window starts HALO before.
 __device__ __forceinline__ void tile_range(int t, const int* cuts, int& lo, int& own_lo, int& own_hi) {
   own_lo = cuts[t];
   own_hi = cuts[t + 1];
   lo = own_lo;
 }
-
-// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a contiguous run of
-// tiles (z fastest) so that tiles sharing halo rows meet in the same L2.
-__device__ __forceinline__ int xcd_tile(int b, int nblk_pad) { return (b & 7) * (nblk_pad >> 3) + (b >> 3); }
 
 __device__ __forceinline__ double warp_max(double v) {
 #pragma unroll
@@ -105,8 +69,8 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long* d
   }
 }
 
-// NF staged fields: rho, Vr, Vt, Vz, T, p, mu, kappa (kappa only with LAD; it equals mu
-// otherwise, AssignValues ModAssign.f90:22).
+// NF staged fields: rho, Vr, Vt, Vz, T, p, mu, kappa, T**0.666 (the last two only with LAD:
+// without it kappa = mu = T**0.666, AssignValues ModAssign.f90:19-22).
 // INT: the tile list holds interior tiles only -- every owned point and every window row of
 // the owned columns takes the interior stencils, and no owned point is on the axis or on a
 // boundary patch -- so the metadata word, the closures, the patches and the axis fix are
@@ -119,7 +83,8 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
   constexpr bool LAD = MODE == MODE_LAD;
   constexpr bool VISC = MODE != MODE_INVISCID;
   constexpr bool SC = MODE == MODE_SC || MODE == MODE_LAD;
-  constexpr int NF = LAD ? 8 : 7;
+  constexpr int NF = LAD ? 9 : 7;
+  constexpr int FK = 7, FM = 8;  // LAD: kappa and the physical mu
   __shared__ double sF[NF][WR][WZ];
   __shared__ double sD[2][WR][TZ];
   __shared__ double red[STAGE_THREADS / 64];
@@ -161,10 +126,11 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
     const int k = threadIdx.x + u * STAGE_THREADS;
     if (k >= WZ * WR) break;
     const int wz = k % WZ, wr = k / WZ;
-    double p = 0.0, mu = 0.0, kap = 0.0;
+    double p = 0.0, mu = 0.0, kap = 0.0, mphys = 0.0;
     if (inside[u]) {
       p = win[u][0] * win[u][4];
       mu = pow(win[u][4], 0.666);
+      mphys = mu;
       kap = mu;
       if (LAD) {
         const long long gi = (long long)(i0 - HALO + wz) + (long long)pitch * (j0 - HALO + wr);
@@ -176,7 +142,10 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
     for (int f = 0; f < 5; ++f) sF[f][wr][wz] = win[u][f];
     sF[5][wr][wz] = p;
     sF[6][wr][wz] = mu;
-    if (LAD) sF[NF - 1][wr][wz] = kap;
+    if (LAD) {
+      sF[FK][wr][wz] = kap;
+      sF[FM][wr][wz] = mphys;
+    }
   }
   __syncthreads();
 
@@ -296,9 +265,9 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
       d.dmudr = d1_sum(rr, &sF[6][wr][wz], WZ) * mr;
       double kappa = mu;
       if (LAD) {
-        kappa = sF[NF - 1][wr][wz];
-        d.dkappadz = d1_sum(zr, &sF[NF - 1][wr][wz], 1) * mz;
-        d.dkappadr = d1_sum(rr, &sF[NF - 1][wr][wz], WZ) * mr;
+        kappa = sF[FK][wr][wz];
+        d.dkappadz = d1_sum(zr, &sF[FK][wr][wz], 1) * mz;
+        d.dkappadr = d1_sum(rr, &sF[FK][wr][wz], WZ) * mr;
       } else {
         d.dkappadz = d.dmudz;
         d.dkappadr = d.dmudr;
@@ -323,7 +292,7 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
         if (SC) {
           // beta = (3/5) T**0.666 (+ beta*), lambda = beta - (2/3) mu (ModAssign.f90:21-23,
           // ModShockCapturing.f90:249-250); without the artificial terms mu is T**0.666 itself
-          const double mu_phys = LAD ? pow(q[4], 0.666) : mu;
+          const double mu_phys = LAD ? sF[FM][wr][wz] : mu;
           double beta = (3.0 / 5.0) * mu_phys;
           if (LAD) beta = beta + a.lad[gi + fs];
           lambda = beta - (2.0 / 3.0) * mu;
@@ -452,9 +421,11 @@ __global__ void cfl_finalize_kernel(StepScalars* sc, double* log, long long log_
   sc->nlog = sc->nlog + 1;
 }
 
-void upload_tables(const double sbp1[9][6], const double sbp2[9][6], hipStream_t s) {
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sbp1), sbp1, sizeof(double) * 54, 0, hipMemcpyHostToDevice, s);
-  (void)hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sbp2), sbp2, sizeof(double) * 54, 0, hipMemcpyHostToDevice, s);
+hipError_t upload_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s) {
+  hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sbp1), sbp1, sizeof(double) * 54, 0, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sbp2), sbp2, sizeof(double) * 54, 0, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_tg), tg, sizeof(double) * 99, 0, hipMemcpyHostToDevice, s);
+  return e;
 }
 
 static int pad8(int n) { return (n + 7) / 8 * 8; }

@@ -1,0 +1,401 @@
+// gfx950 kernels of the localized artificial diffusivity (ComputeArtTransCoeffs,
+// ModShockCapturing.f90:5-106 with PerformTGfilter, ModTGfilter.f90:5-95):
+//
+//   lad_strain_kernel   |S|, dilatation and the Ducros-type switch f_sw at every point
+//   lad_source_kernel   D2(D2(.)) of |S|, dil and T in xi and in eta; the three quantities
+//   lad_zfilter_kernel  the z filter, applied in place by the reference, i.e. the causal
+//                       recursion y_i = sum_{m<0} c_m y_{i+m} + sum_{m>=0} c_m x_{i+m}
+//   lad_rfilter_kernel  the r filter, C Re (Pr) scaling and the tanh ramp -> mu*, beta*, kappa*
+//
+// Build with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "ajet_kernels.h"
+#include "ajet_stencil.h"
+
+namespace ajet {
+
+// ---- |S|, dil, f_sw (ComputeDilatation, ComputeMgnStrRate ModShockCapturing.f90:108-159, and
+// the switch of ComputeTobeFilteredQuantities :196-202).  Same tiles and 2-point window as the
+// stage kernel; the first derivatives take the patch closures and are zero at blanked points.
+// INT: the tile list holds tiles whose points and stencil rows are all interior (see
+// classify_lad_tiles), so no metadata is read.
+template <bool INT>
+__global__ __launch_bounds__(256) void lad_strain_kernel(LadArgs a, const int* __restrict__ tiles, int ntiles) {
+  __shared__ double sVr[WR][WZ];
+  __shared__ double sVz[WR][WZ];
+  const Geometry& g = a.g;
+  const int nz = g.nz, nr = g.nr, pitch = g.pitch;
+  const long long fs = g.fstride;
+  const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (li >= ntiles) return;
+  const int t = tiles[li];
+  const int tzi = t % g.ntz, tri = t / g.ntz;
+  const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
+  for (int k = threadIdx.x; k < WZ * WR; k += blockDim.x) {
+    const int wz = k % WZ, wr = k / WZ;
+    const int i = i0 - HALO + wz, j = j0 - HALO + wr;
+    // outside the grid: load point 0 (a valid address) and zero it, so no load is branched
+    const bool in = i >= 0 && i < nz && j >= 0 && j < nr;
+    const long long gi = in ? (long long)i + (long long)pitch * j : 0;
+    const double vr = a.q[gi + fs], vz = a.q[gi + 3 * fs];
+    sVr[wr][wz] = in ? vr : 0.0;
+    sVz[wr][wz] = in ? vz : 0.0;
+  }
+  __syncthreads();
+  const int tz = threadIdx.x % TZ;
+  const double c13 = 1.0 / 3.0;
+  const double eps = (double)1e-15f;  // a default-REAL literal (ModShockCapturing.f90:202)
+  for (int ty = threadIdx.x / TZ; ty < TR; ty += blockDim.x / TZ) {
+    const int i = i0 + tz, j = j0 + ty;
+    if (i >= i1 || j >= j1) continue;
+    const long long gi = (long long)i + (long long)pitch * j;
+    const uint32_t meta = INT ? (ROW_INTERIOR | (ROW_INTERIOR << 4)) : a.meta[gi];
+    const uint32_t zr = meta & 15u, rr = (meta >> 4) & 15u;
+    const int wz = tz + HALO, wr = ty + HALO;
+    const double mz = a.dxidz[i], mr = a.detadr[j];
+    const double dVrdz = d1_sum(zr, &sVr[wr][wz], 1) * mz;
+    const double dVzdz = d1_sum(zr, &sVz[wr][wz], 1) * mz;
+    const double dVrdr = d1_sum(rr, &sVr[wr][wz], WZ) * mr;
+    const double dVzdr = d1_sum(rr, &sVz[wr][wz], WZ) * mr;
+    const double Vr = sVr[wr][wz];
+    // the axis row takes Vr/r -> dVr/dr (ModShockCapturing.f90:116-120, 146-150)
+    double dil, S22;
+    if (!INT && j == 0) {
+      dil = 2.0 * dVrdr + dVzdz;
+      S22 = dVrdr - c13 * dil;
+    } else {
+      const double vbr = Vr / a.rc[j];
+      dil = (dVrdr + vbr) + dVzdz;
+      S22 = vbr - c13 * dil;
+    }
+    const double S11 = dVrdr - c13 * dil;
+    const double S13 = 0.5 * (dVrdz + dVzdr);
+    const double S33 = dVzdz - c13 * dil;
+    // the nine squares summed in order; S12, S21, S23, S32 are zero and add exact zeros
+    const double S = sqrt((((S11 * S11 + S13 * S13) + S22 * S22) + S13 * S13) + S33 * S33);
+    const double vort = fabs(dVrdz - dVzdr);
+    const double md = -1.0 * dil;
+    const double H = md < 0.0 ? 0.0 : 1.0;
+    const double fsw = H * (dil * dil) / ((dil * dil + vort * vort) + eps);
+    a.A[gi] = S;
+    a.A[gi + fs] = dil;
+    a.A[gi + 2 * fs] = fsw;
+  }
+}
+
+// ---- D2(D2(.)) along xi and eta of |S|, dil and T, and the three quantities
+// (ComputeTobeFilteredQuantities, ModShockCapturing.f90:162-222).  The second stencil of each
+// pair reads the first one at up to 2 points beyond the tile (closure strips never straddle a
+// tile edge), and that one reads its argument at up to 5 beyond, so |S|, dil and T are staged
+// with a 7-point halo.  A second derivative at a blanked point or outside the grid is zero.
+constexpr int LH = 7;
+constexpr int LWZ = TZ + 2 * LH;
+constexpr int LWR = TR + 2 * LH;
+
+template <bool INT>
+__global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* __restrict__ tiles, int ntiles) {
+  __shared__ double sB[3][LWR][LWZ];   // |S|, dil, T
+  __shared__ double sZ[3][TR][TZ + 4];  // D2 along xi, columns i0-2 .. i0+TZ+1
+  __shared__ double sR[3][TR + 4][TZ];  // D2 along eta, rows j0-2 .. j0+TR+1
+  __shared__ uint8_t cZ[TR][TZ + 4];    // z row codes where the D2 along xi is taken
+  __shared__ uint8_t cR[TR + 4][TZ];    // r row codes where the D2 along eta is taken
+  const Geometry& g = a.g;
+  const int nz = g.nz, nr = g.nr, pitch = g.pitch;
+  const long long fs = g.fstride;
+  const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (li >= ntiles) return;
+  const int t = tiles[li];
+  const int tzi = t % g.ntz, tri = t / g.ntz;
+  const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
+  for (int k = threadIdx.x; k < LWZ * LWR; k += blockDim.x) {
+    const int wz = k % LWZ, wr = k / LWZ;
+    const int i = i0 - LH + wz, j = j0 - LH + wr;
+    const bool in = i >= 0 && i < nz && j >= 0 && j < nr;
+    const long long gi = in ? (long long)i + (long long)pitch * j : 0;
+    const double s = a.A[gi], d = a.A[gi + fs], T = a.q[gi + 4 * fs];
+    sB[0][wr][wz] = in ? s : 0.0;
+    sB[1][wr][wz] = in ? d : 0.0;
+    sB[2][wr][wz] = in ? T : 0.0;
+  }
+  if (!INT) {
+    // codes of the points where a second derivative is taken; outside the grid it is zero,
+    // which the blank code gives
+    for (int k = threadIdx.x; k < TR * (TZ + 4); k += blockDim.x) {
+      const int c = k % (TZ + 4), ty = k / (TZ + 4);
+      const int i = i0 - 2 + c, j = j0 + ty;
+      const bool in = j < j1 && i >= 0 && i < nz;
+      const uint32_t m = a.meta[in ? (long long)i + (long long)pitch * j : 0];
+      cZ[ty][c] = (uint8_t)(in ? (m & 15u) : ROW_BLANK);
+    }
+    for (int k = threadIdx.x; k < (TR + 4) * TZ; k += blockDim.x) {
+      const int c = k % TZ, w = k / TZ;
+      const int i = i0 + c, j = j0 - 2 + w;
+      const bool in = i < i1 && j >= 0 && j < nr;
+      const uint32_t m = a.meta[in ? (long long)i + (long long)pitch * j : 0];
+      cR[w][c] = (uint8_t)(in ? ((m >> 4) & 15u) : ROW_BLANK);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < TR * (TZ + 4); k += blockDim.x) {
+    const int c = k % (TZ + 4), ty = k / (TZ + 4);
+    const uint32_t zr = INT ? ROW_INTERIOR : cZ[ty][c];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) sZ[f][ty][c] = d2_sum(zr, &sB[f][ty + LH][c - 2 + LH], 1);
+  }
+  for (int k = threadIdx.x; k < (TR + 4) * TZ; k += blockDim.x) {
+    const int c = k % TZ, w = k / TZ;
+    const uint32_t rr = INT ? ROW_INTERIOR : cR[w][c];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) sR[f][w][c] = d2_sum(rr, &sB[f][w - 2 + LH][c + LH], LWZ);
+  }
+  __syncthreads();
+  const int tz = threadIdx.x % TZ;
+  for (int ty = threadIdx.x / TZ; ty < TR; ty += blockDim.x / TZ) {
+    const int i = i0 + tz, j = j0 + ty;
+    if (i >= i1 || j >= j1) continue;
+    const long long gi = (long long)i + (long long)pitch * j;
+    const uint32_t zr = INT ? ROW_INTERIOR : cZ[ty][tz + 2], rr = INT ? ROW_INTERIOR : cR[ty + 2][tz];
+    double dz4[3], dr4[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      dz4[f] = d2_sum(zr, &sZ[f][ty][tz + 2], 1);
+      dr4[f] = d2_sum(rr, &sR[f][ty + 2][tz], TZ);
+    }
+    const double rho = a.q[gi];
+    const double fsw = a.A[gi + 2 * fs];
+    const double T = sB[2][ty + LH][tz + LH];
+    const double dz = a.dzl[i], dr = a.drl[j];
+    const double dz2 = dz * dz, dr2 = dr * dr;
+    a.B[gi] = rho * fabs(dz4[0] * dz2 + dr4[0] * dr2);
+    a.B[gi + fs] = rho * fsw * fabs(dz4[1] * dz2 + dr4[1] * dr2);
+    a.B[gi + 2 * fs] = rho / (a.gam * sqrt(T)) * fabs(dz4[2] * dz + dr4[2] * dr);
+  }
+}
+
+// ---- the z filter, which the reference applies in place (PerformTGfilter with var = out,
+// ModShockCapturing.f90:63, 69, 75): walking i upwards, each point reads the already filtered
+// values before it and the unfiltered ones from itself on.  The recursion forgets its start at
+// the rate of the largest root of z^4 - c3 z^3 - c2 z^2 - c1 z - c0 (c = interior weights
+// before the centre), 0.467, so a line is cut into segments that each start ZWARM points early
+// from the unfiltered values: the start error is damped by 0.467^48 < 1.3e-16 before the first
+// point a segment writes.  A segment within ZWARM of the line start runs the exact recursion
+// from i = 1.
+//
+// One wave walks 64 lines (one lane each) through its segment.  The columns stream through an
+// LDS ring of ZRING columns in chunks of ZCH: while the lanes filter one chunk, the loads of a
+// chunk two ahead are in flight in registers, and each finished chunk is written back with
+// 128-byte rows.  The walk keeps the last four filtered values and the next five inputs in
+// registers (and reads the ring 9 points ahead), so an interior step waits only on its own
+// arithmetic; closure rows read their 11
+// points from the ring (up to 10 behind or 10 ahead).  At chunk k the ring holds the columns
+// from ib - 10 + 16 (k - 1) (ib - 10 at k = 0) up to ib + 37 + 16 k, which covers every read
+// of the chunk's steps i = ib + 16 k .. ib + 16 k + 15 (columns i - 10 .. i + 10).
+constexpr int ZWARM = 48;
+constexpr int ZLINES = 64;
+constexpr int ZCH = 16;
+constexpr int ZRING = 64;
+
+__host__ __device__ constexpr int zfilter_segment(int nz) {
+  // about 64 segments per line (several waves per CU), a multiple of the chunk, at least 64
+  return ((nz + 63) / 64 + ZCH - 1) / ZCH * ZCH < 64 ? 64 : ((nz + 63) / 64 + ZCH - 1) / ZCH * ZCH;
+}
+
+__global__ __launch_bounds__(ZLINES) void lad_zfilter_kernel(LadArgs a, int seg) {
+  __shared__ double ring[ZRING][ZLINES + 1];
+  __shared__ uint8_t code[ZRING][ZLINES];
+  const Geometry& g = a.g;
+  const int nz = g.nz, nr = g.nr, pitch = g.pitch;
+  const long long fs = g.fstride;
+  const int S0 = (int)blockIdx.x * seg;
+  const int S1 = min(nz, S0 + seg);
+  const int jb = (int)blockIdx.y * ZLINES;
+  const int f = (int)blockIdx.z;
+  const double* __restrict__ x = a.B + f * fs;
+  double* __restrict__ y = a.A + f * fs;
+  const int lane = threadIdx.x;
+  const int c16 = lane & (ZCH - 1), l4 = lane / ZCH;  // chunk transfers: 4 lines x 16 columns per instruction
+  const int ib = max(0, S0 - ZWARM);                   // first point the walk filters
+  const int P0 = ib - 10;                              // column of ring slot 0: the farthest a closure row reads back
+  // one chunk of columns [q, q+ZCH) of the 64 lines: loads into registers, then into the ring
+  double v[ZLINES / 4];
+  uint32_t cz[ZLINES / 4];
+  auto load_chunk = [&](int q) {
+#pragma unroll
+    for (int r = 0; r < ZLINES / 4; ++r) {
+      const int j = jb + 4 * r + l4, i = q + c16;
+      const bool in = j < nr && i >= 0 && i < nz;
+      const long long gi = in ? (long long)i + (long long)pitch * j : 0;
+      const double xv = x[gi];
+      const uint32_t m = a.meta[gi];
+      v[r] = in ? xv : 0.0;
+      cz[r] = in ? (m & 15u) : ROW_BLANK;
+    }
+  };
+  auto store_chunk = [&](int q) {
+    const int slot = (q + c16 - P0) & (ZRING - 1);
+#pragma unroll
+    for (int r = 0; r < ZLINES / 4; ++r) {
+      ring[slot][4 * r + l4] = v[r];
+      code[slot][4 * r + l4] = (uint8_t)cz[r];
+    }
+  };
+  for (int q = P0; q < P0 + 3 * ZCH; q += ZCH) {
+    load_chunk(q);
+    store_chunk(q);
+  }
+  __syncthreads();
+  int i = ib;
+  int sl = (i - P0) & (ZRING - 1);
+  // filtered values at i-1..i-4 (zeros before the line start, else the unfiltered values the
+  // warm-up forgets), unfiltered values at i..i+8 and row codes at i..i+4: each LDS read is
+  // issued several steps before its value is needed
+  double y1 = ring[(sl - 1) & (ZRING - 1)][lane], y2 = ring[(sl - 2) & (ZRING - 1)][lane];
+  double y3 = ring[(sl - 3) & (ZRING - 1)][lane], y4 = ring[(sl - 4) & (ZRING - 1)][lane];
+  double X[9];
+  uint32_t Z[5];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) X[m] = ring[(sl + m) & (ZRING - 1)][lane];
+#pragma unroll
+  for (int m = 0; m < 5; ++m) Z[m] = code[(sl + m) & (ZRING - 1)][lane];
+  for (int k = 0; ib + k * ZCH < S1; ++k) {
+    const int cb = ib + k * ZCH;  // first point of this chunk
+    const int q = P0 + 3 * ZCH + k * ZCH;
+    load_chunk(q);                // two chunks ahead of the walk
+    const int n = min(ZCH, S1 - cb);
+    auto step = [&]() {
+      sl = (i - P0) & (ZRING - 1);
+      const double xn = ring[(sl + 9) & (ZRING - 1)][lane];
+      const uint32_t zn = code[(sl + 5) & (ZRING - 1)][lane];
+      const uint32_t zr = Z[0];
+      // the interior filter always; closure rows and blanked points (a few per line) replace it
+      double w = (((((((TG0 * y4 + TG1 * y3) + TG2 * y2) + TG3 * y1) + TG4 * X[0]) + TG3 * X[1]) + TG2 * X[2]) +
+                  TG1 * X[3]) + TG0 * X[4];
+      if (zr != ROW_INTERIOR) {
+        if (zr == ROW_BLANK) {
+          w = X[0];  // blanked points keep their value (ModTGfilter.f90:17)
+        } else {
+          // closure row: 11 ring columns from base = -row (rows 0-3) or -row-2 (rows 5-8)
+          const int base = zr < 4 ? -(int)zr : -(int)zr - 2;
+          double acc = 0.0;
+#pragma unroll
+          for (int m = 0; m < 11; ++m) acc = acc + c_tg[zr][m] * ring[(sl + base + m) & (ZRING - 1)][lane];
+          w = acc;
+        }
+      }
+      ring[sl][lane] = w;
+      y4 = y3;
+      y3 = y2;
+      y2 = y1;
+      y1 = w;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) X[m] = X[m + 1];
+      X[8] = xn;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) Z[m] = Z[m + 1];
+      Z[4] = zn;
+      ++i;
+    };
+    if (n == ZCH) {
+      // a whole chunk, unrolled: the register queues rotate by renaming
+#pragma unroll
+      for (int t = 0; t < ZCH; ++t) step();
+    } else {
+      for (int t = 0; t < n; ++t) step();
+    }
+    __syncthreads();
+    // write back the chunk's points that belong to the segment
+#pragma unroll
+    for (int r = 0; r < ZLINES / 4; ++r) {
+      const int j = jb + 4 * r + l4, ii = cb + c16;
+      if (j < nr && ii >= S0 && ii < S1 && c16 < n)
+        y[(long long)ii + (long long)pitch * j] = ring[(ii - P0) & (ZRING - 1)][4 * r + l4];
+    }
+    store_chunk(q);  // overwrites the columns 64 behind q, already written back
+    __syncthreads();
+  }
+}
+
+// ---- the r filter (PerformTGfilter along eta into mu*, beta*, kappa*), then the scaling by
+// C Re (C_kappa Re Pr) and the ramp (ModShockCapturing.f90:64-92).  Blanked points are left
+// out by the filter and keep zero.  Each thread walks RROWS rows of one column with the
+// interior window in registers; closure rows read their 11 points directly.
+constexpr int RROWS = 8;
+
+__global__ __launch_bounds__(256) void lad_rfilter_kernel(LadArgs a) {
+  const Geometry& g = a.g;
+  const int nz = g.nz, nr = g.nr, pitch = g.pitch;
+  const long long fs = g.fstride;
+  const int i = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
+  const int jr = ((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * RROWS;
+  if (i >= nz || jr >= nr) return;
+  const double ramp = a.ramp[i];
+  uint32_t rr[RROWS];
+#pragma unroll
+  for (int m = 0; m < RROWS; ++m) {
+    const bool in = jr + m < nr;
+    const uint32_t mw = a.meta[(long long)i + (long long)pitch * (in ? jr + m : 0)];
+    rr[m] = in ? (mw >> 4) & 15u : ROW_BLANK;
+  }
+  const double C[3] = {a.cmu, a.cbeta, a.ckappa};
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const double* col = a.A + f * fs + i;
+    double w[RROWS + 8];  // rows jr-4 .. jr+RROWS+3, zero outside the grid
+#pragma unroll
+    for (int m = 0; m < RROWS + 8; ++m) {
+      const int jj = jr - 4 + m;
+      const bool in = jj >= 0 && jj < nr;
+      const double wv = col[(long long)pitch * (in ? jj : 0)];
+      w[m] = in ? wv : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < RROWS; ++m) {
+      const int j = jr + m;
+      if (j >= nr) break;
+      double v = 0.0;
+      if (rr[m] == ROW_INTERIOR) {
+        v = (((((((TG0 * w[m] + TG1 * w[m + 1]) + TG2 * w[m + 2]) + TG3 * w[m + 3]) + TG4 * w[m + 4]) +
+               TG3 * w[m + 5]) + TG2 * w[m + 6]) + TG1 * w[m + 7]) + TG0 * w[m + 8];
+        v = C[f] * v;
+        v = v * ramp;
+      } else if (rr[m] != ROW_BLANK) {
+        const int lo = rr[m] < 4 ? j - (int)rr[m] : j - (int)rr[m] - 2;
+        v = 0.0;
+        for (int t = 0; t < 11; ++t) {
+          const int jj = lo + t;
+          const bool in = jj >= 0 && jj < nr;
+          const double xv = col[(long long)pitch * (in ? jj : 0)];
+          v = v + c_tg[rr[m]][t] * (in ? xv : 0.0);
+        }
+        v = C[f] * v;
+        v = v * ramp;
+      }
+      a.out[(long long)i + (long long)pitch * j + f * fs] = v;
+    }
+  }
+}
+
+hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s) {
+  hipError_t e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sbp1), sbp1, sizeof(double) * 54, 0, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_sbp2), sbp2, sizeof(double) * 54, 0, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyToSymbolAsync(HIP_SYMBOL(c_tg), tg, sizeof(double) * 99, 0, hipMemcpyHostToDevice, s);
+  return e;
+}
+
+hipError_t launch_lad(const LadArgs& a, hipStream_t s) {
+  const Geometry& g = a.g;
+  auto grid = [](int n) { return dim3((unsigned)((n + 7) / 8 * 8)); };
+  if (a.n_int > 0) hipLaunchKernelGGL(lad_strain_kernel<true>, grid(a.n_int), dim3(256), 0, s, a, a.int_tiles, a.n_int);
+  if (a.n_bnd > 0) hipLaunchKernelGGL(lad_strain_kernel<false>, grid(a.n_bnd), dim3(256), 0, s, a, a.bnd_tiles, a.n_bnd);
+  if (a.n_int > 0) hipLaunchKernelGGL(lad_source_kernel<true>, grid(a.n_int), dim3(256), 0, s, a, a.int_tiles, a.n_int);
+  if (a.n_bnd > 0) hipLaunchKernelGGL(lad_source_kernel<false>, grid(a.n_bnd), dim3(256), 0, s, a, a.bnd_tiles, a.n_bnd);
+  const int seg = zfilter_segment(g.nz);
+  hipLaunchKernelGGL(lad_zfilter_kernel, dim3((g.nz + seg - 1) / seg, (g.nr + ZLINES - 1) / ZLINES, 3), dim3(ZLINES),
+                     0, s, a, seg);
+  hipLaunchKernelGGL(lad_rfilter_kernel, dim3((g.nz + 63) / 64, (g.nr + 4 * RROWS - 1) / (4 * RROWS)), dim3(256), 0,
+                     s, a);
+  return hipGetLastError();
+}
+
+}  // namespace ajet

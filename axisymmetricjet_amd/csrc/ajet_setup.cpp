@@ -284,6 +284,29 @@ void classify_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, 
     }
 }
 
+// Tiles of the artificial-diffusivity kernels that need no metadata: every point where the
+// tile takes a second derivative -- the owned rows over the owned columns +-2 (along xi), the
+// owned columns over the owned rows +-2 (along eta) -- is inside the grid and takes the
+// interior stencil in that direction.  That also makes every owned point interior in z and r
+// and keeps the axis row out.
+void classify_lad_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, std::vector<int>& boundary) {
+  interior.clear();
+  boundary.clear();
+  const int ntz = (int)s.zcuts.size() - 1, ntr = (int)s.rcuts.size() - 1;
+  for (int tr = 0; tr < ntr; ++tr)
+    for (int tz = 0; tz < ntz; ++tz) {
+      const int i0 = s.zcuts[tz], i1 = s.zcuts[tz + 1], j0 = s.rcuts[tr], j1 = s.rcuts[tr + 1];
+      bool ok = i0 - 2 >= 0 && i1 + 2 <= nz && j0 - 2 >= 0 && j1 + 2 <= nr;
+      for (int j = j0; ok && j < j1; ++j)
+        for (int i = i0 - 2; ok && i < i1 + 2; ++i)
+          if ((s.meta[(size_t)i + (size_t)nz * j] & 15u) != ROW_INTERIOR) ok = false;
+      for (int j = j0 - 2; ok && j < j1 + 2; ++j)
+        for (int i = i0; ok && i < i1; ++i)
+          if (((s.meta[(size_t)i + (size_t)nz * j] >> 4) & 15u) != ROW_INTERIOR) ok = false;
+      (ok ? interior : boundary).push_back(tr * ntz + tz);
+    }
+}
+
 int build_tiling(Setup& s, int nz, int nr, std::string& err) {
   const int BIG = 1 << 30;
   std::vector<int> zlo(nz, BIG), zhi(nz, -BIG), rlo(nr, BIG), rhi(nr, -BIG);
@@ -311,6 +334,32 @@ int build_tiling(Setup& s, int nz, int nr, std::string& err) {
     return AJET_ETILE;
   }
   return AJET_OK;
+}
+
+// Grid spacings of ComputeTobeFilteredQuantities (ModShockCapturing.f90:169-187), centred
+// inside and one-sided at the two ends, and the ramp (1 + tanh(5 (z - 10))) / 2 of
+// ComputeArtTransCoeffs (ModShockCapturing.f90:85).  The grid is a tensor product, so all
+// three are 1-D; they are computed here, on the host, with the host's tanh.
+void lad_tables(const Setup& s, int nz, int nr, std::vector<double>& dz, std::vector<double>& dr,
+                std::vector<double>& ramp) {
+  dz.assign((size_t)nz, 0.0);
+  dr.assign((size_t)nr, 0.0);
+  ramp.assign((size_t)nz, 0.0);
+  for (int i = 0; i < nz; ++i) {
+    double d = 0.0;
+    if (i >= 1 && i <= nz - 2) d = (s.zc[i + 1] - s.zc[i - 1]) / 2.0;
+    if (i == 0 && nz > 1) d = s.zc[1] - s.zc[0];
+    if (i == nz - 1 && nz > 1) d = s.zc[nz - 1] - s.zc[nz - 2];
+    dz[i] = d;
+    ramp[i] = (1.0 + std::tanh(5.0 * (s.zc[i] - 10.0))) / 2.0;
+  }
+  for (int j = 0; j < nr; ++j) {
+    double d = 0.0;
+    if (j >= 1 && j <= nr - 2) d = (s.rc[j + 1] - s.rc[j - 1]) / 2.0;
+    if (j == 0 && nr > 1) d = s.rc[1] - s.rc[0];
+    if (j == nr - 1 && nr > 1) d = s.rc[nr - 1] - s.rc[nr - 2];
+    dr[j] = d;
+  }
 }
 
 }  // namespace ajet

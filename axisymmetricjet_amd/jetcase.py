@@ -39,6 +39,13 @@ class JetCase:
     pr: float
 
 
+# LAD constants C_mu, C_beta, C_kappa of the shock-capturing runs (the reference ships 0, 0, 0).
+# C_beta is a tenth of the usual 1.75: the reference's time step ignores the artificial
+# viscosities, and with 1.75 the seeded grid-scale noise drives beta* to ~70 at the outflow
+# corner of the 4400 x 640 jet and the run diverges within 5 steps; with 0.175 it stays bounded.
+LAD_C = (0.002, 0.175, 0.01)
+
+
 def stretched_r(nr: int, r_max: float = 8.0, delta: float = 2.0) -> np.ndarray:
     """tanh stretching on [0, r_max], fine at the axis and the shear layer, r[0] = 0."""
     eta = np.arange(nr, dtype=np.float64) / (nr - 1)

@@ -1,17 +1,23 @@
 #!/usr/bin/env python3
 """RK4 time-step throughput of the axisymmetric jet on one MI355X.
 
-Workload: the plain FP64 jet at 4400 x 640 (z x r), Mach 1.5, Re 1e4, Pr 0.7, seeded
+Workload: the FP64 jet at 4400 x 640 (z x r), Mach 1.5, Re 1e4, Pr 0.7, seeded
 hyperbolic-tangent top-hat initial condition, SAT inflow / NSCBC top and outflow / axis
-boundaries, CFL 0.5.  A step is one {ComputeTimeStep; RK4} iteration (ModSolver.f90:122-125)
-over the whole grid, with the state resident in HBM.  Prints one JSON line.
+boundaries, CFL 0.5, with shock capturing and filtering on (ShockCapturing = 1: the localized
+artificial diffusivity with C_mu, C_beta, C_kappa = jetcase.LAD_C, the TG filters in z and r).
+A step is one {ComputeTimeStep; RK4} iteration (ModSolver.f90:122-125) over the whole grid,
+with the state resident in HBM.  The plain step (ShockCapturing = 0) is timed beside it.
+Prints one JSON line.
 
   value      grid-point updates/s = nz * nr * steps / time of the timed region
-  roofline   the four RK-stage kernel launches of a step against the HBM peak: the
-             algorithmic traffic is 640 B per grid-point update (5 FP64 variables; stage 1
-             reads q^n and writes q1, A; stage 2 reads q1, q^n and writes q2, S; stage 3 reads
-             q2, q^n, S and writes q3, S; stage 4 reads q3, A, S and writes q^{n+1}), divided by
-             the summed average durations of the four stage launches measured with HIP events
+  roofline   the four RK stages of a step (each: the artificial-diffusivity launches, then the
+             stage kernel) against the HBM peak: the algorithmic traffic is 640 B per grid-point
+             update (5 FP64 variables; stage 1 reads q^n and writes q1, A; stage 2 reads q1, q^n
+             and writes q2, S; stage 3 reads q2, q^n, S and writes q3, S; stage 4 reads q3, A, S
+             and writes q^{n+1}), divided by the summed average durations of the four stages
+             measured with HIP events on the library's stream.  traffic is the HBM bytes per
+             step measured with rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE) for the same
+             command, kept in MEASURED_TRAFFIC
   cpu_baseline  the C restatement (oracle) of the same step on the host cores, OpenMP over
              grid lines, on a bounded sample of steps of the same grid
 """
@@ -29,14 +35,20 @@ sys.path.insert(0, ROOT)
 BYTES_PER_UPDATE = 640
 HBM_PEAK_GBPS = 8000.0
 
+# HBM bytes per grid-point update of a whole step, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE over the same workload (FETCH scaled by 2.0 and WRITE by 1.0, the factors a 1 GiB
+# device copy reads on the same counters); None where not measured.
+MEASURED_TRAFFIC = {"plain": 723.4, "lad": None}
+
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(case, seconds: float, threads: int):
+def cpu_baseline(case, seconds: float, threads: int, lad_c):
     from oracle import pyoracle as po
-    o = po.Oracle(case, threads=threads)
+    c = lad_c or (0.0, 0.0, 0.0)
+    o = po.Oracle(case, shock=1 if lad_c else 0, c_mu=c[0], c_beta=c[1], c_kappa=c[2], threads=threads)
     o.run(1)  # warm-up (first touch of the ~1.6 GB of halo fields)
     n, t0 = 0, time.perf_counter()
     while True:
@@ -56,6 +68,9 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nz", type=int, default=4400)
     ap.add_argument("--nr", type=int, default=640)
+    ap.add_argument("--method", choices=["lad", "plain"], default="lad",
+                    help="lad: ShockCapturing = 1 with the LAD constants; plain: ShockCapturing = 0")
+    ap.add_argument("--other-steps", type=int, default=200, help="steps of the other method timed beside")
     ap.add_argument("--stage-steps", type=int, default=20, help="steps timed launch by launch for the roofline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -85,28 +100,47 @@ def main(argv=None):
     npts = nz * nr
     log(f"building case {nz}x{nr}")
     case = jc.make_jet(nz, nr, seed=1234)
-    ctx = capi.Context(case)
+
+    def context(method):
+        if method == "lad":
+            c = jc.LAD_C
+            return capi.Context(case, shock_capturing=1, C_mu=c[0], C_beta=c[1], C_kappa=c[2])
+        return capi.Context(case)
+
+    def timed_run(ctx, steps, warmup):
+        stream = torch.cuda.ExternalStream(ctx.stream())
+        ctx.run(warmup)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        ctx.run(steps)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        dts, _ = ctx.dt_log()
+        if not (dts[-1] > 0 and dts[-1] < 1.0):
+            raise RuntimeError(f"implausible time step {dts[-1]}")
+        return ms, dts[-1]
+
+    other = "plain" if args.method == "lad" else "lad"
+    others = {}
+    if args.other_steps > 0:
+        octx = context(other)
+        oms, _ = timed_run(octx, args.other_steps, args.warmup)
+        others[other] = {"value": npts * args.other_steps / (oms / 1e3), "ms_per_step": oms / args.other_steps,
+                         "steps": args.other_steps}
+        log(f"{other}: {oms / args.other_steps:.4f} ms/step")
+        octx.close()
+
+    ctx = context(args.method)
     stream = torch.cuda.ExternalStream(ctx.stream())
-    log(f"warm-up {args.warmup} steps")
-    ctx.run(args.warmup)
-    torch.cuda.synchronize()
-
-    log(f"timed region: {args.steps} steps")
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    ctx.run(args.steps)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
-    dts, t = ctx.dt_log()
-    if not (dts[-1] > 0 and dts[-1] < 1.0):
-        raise RuntimeError(f"implausible time step {dts[-1]}")
+    log(f"{args.method}: warm-up {args.warmup} steps, timed region {args.steps} steps")
+    ms, dt_last = timed_run(ctx, args.steps, args.warmup)
     value = npts * args.steps / (ms / 1e3)
-    log(f"{ms / args.steps:.4f} ms/step, {value:.4e} updates/s, dt[-1]={dts[-1]:.6e}")
+    log(f"{ms / args.steps:.4f} ms/step, {value:.4e} updates/s, dt[-1]={dt_last:.6e}")
 
-    # the four stage kernels, each launch bracketed by events on the library's stream
+    # the four stages, each bracketed by events on the library's stream
     ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(4)]
           for _ in range(args.stage_steps)]
     for k in range(args.stage_steps):
@@ -120,16 +154,16 @@ def main(argv=None):
                 for s in range(4)]
     step_kernel_ms = sum(stage_ms)
     achieved = BYTES_PER_UPDATE * npts / (step_kernel_ms / 1e3) / 1e9
-    log("stage kernel avg ms: " + ", ".join(f"{x:.4f}" for x in stage_ms))
+    log("stage avg ms: " + ", ".join(f"{x:.4f}" for x in stage_ms))
 
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle, {threads} threads, ~{args.cpu_seconds:.0f} s")
-        v, n, el = cpu_baseline(case, args.cpu_seconds, threads)
+        v, n, el = cpu_baseline(case, args.cpu_seconds, threads, jc.LAD_C if args.method == "lad" else None)
         cpu = {"value": v, "unit": "grid-point updates/s", "cores": threads, "kind": "port",
-               "sample": f"{n} RK4 steps of the same {nz}x{nr} jet (after 1 warm-up step), {el:.1f} s, "
-                         f"C restatement at -O2 without FMA, OpenMP over grid lines"}
+               "sample": f"{n} RK4 steps ({args.method}) of the same {nz}x{nr} jet (after 1 warm-up step), "
+                         f"{el:.1f} s, C restatement at -O2 without FMA, OpenMP over grid lines"}
         log(f"cpu baseline {v:.4e} updates/s")
 
     out = {
@@ -145,13 +179,17 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"plain FP64 axisymmetric jet RK4 step, {nz}x{nr} (z x r), Mach 1.5, "
-                               f"Re 1e4, Pr 0.7, CFL 0.5, SAT/NSCBC/axis boundaries",
-                   "nz": nz, "nr": nr, "points": npts, "method": "plain"},
+        "config": {"workload": f"FP64 axisymmetric jet RK4 step, {nz}x{nr} (z x r), Mach 1.5, Re 1e4, Pr 0.7, "
+                               f"CFL 0.5, SAT/NSCBC/axis boundaries, "
+                               + ("shock capturing and filtering on (LAD C = %g, %g, %g)" % jc.LAD_C
+                                  if args.method == "lad" else "shock capturing off"),
+                   "nz": nz, "nr": nr, "points": npts, "method": args.method},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": None,
-                     "kernel": "stage_kernel<1..4> (one launch per RK stage)",
+                     "frac": achieved / HBM_PEAK_GBPS,
+                     "traffic": MEASURED_TRAFFIC[args.method] * npts if MEASURED_TRAFFIC[args.method] and (nz, nr) == (4400, 640) else None,
+                     "kernel": "one RK stage: artificial-diffusivity launches (lad) + stage_kernel<stage>",
                      "stage_ms": stage_ms, "bytes_per_update": BYTES_PER_UPDATE},
+        "other_methods": others,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
