@@ -183,9 +183,9 @@ __global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* _
 // from i = 1.
 //
 // One wave walks 64 lines (one lane each) through its segment.  The columns stream through an
-// LDS ring of ZRING columns in chunks of ZCH: while the lanes filter one chunk, the loads of a
-// chunk two ahead are in flight in registers, and each finished chunk is written back with
-// 128-byte rows.  The walk keeps the last four filtered values and the next five inputs in
+// LDS ring of ZRING columns in chunks of ZCH: while the lanes filter chunk k, the loads of
+// chunk k + 4 are in flight and chunk k + 3 waits in registers, and each finished chunk is
+// written back with 128-byte rows.  The walk keeps the last four filtered values and the next five inputs in
 // registers (and reads the ring 9 points ahead), so an interior step waits only on its own
 // arithmetic; closure rows read their 11
 // points from the ring (up to 10 behind or 10 ahead).  At chunk k the ring holds the columns
@@ -201,9 +201,44 @@ __host__ __device__ constexpr int zfilter_segment(int nz) {
   return ((nz + 63) / 64 + ZCH - 1) / ZCH * ZCH < 64 ? 64 : ((nz + 63) / 64 + ZCH - 1) / ZCH * ZCH;
 }
 
+// Walk state of one lane: the last four filtered values and the next nine unfiltered ones.
+struct ZWalk {
+  double y1, y2, y3, y4;
+  double X[9];
+};
+
+__device__ __forceinline__ double tg_interior(const ZWalk& st) {
+  return (((((((TG0 * st.y4 + TG1 * st.y3) + TG2 * st.y2) + TG3 * st.y1) + TG4 * st.X[0]) + TG3 * st.X[1]) +
+           TG2 * st.X[2]) + TG1 * st.X[3]) + TG0 * st.X[4];
+}
+
+__device__ __forceinline__ void zwalk_advance(ZWalk& st, double w, double xn) {
+  st.y4 = st.y3;
+  st.y3 = st.y2;
+  st.y2 = st.y1;
+  st.y1 = w;
+#pragma unroll
+  for (int m = 0; m < 8; ++m) st.X[m] = st.X[m + 1];
+  st.X[8] = xn;
+}
+
+// 16 steps of a chunk whose points all take the interior filter.  Walk chunk k starts in ring
+// slot (10 + 16 k) mod 64, so KM = k mod 4 fixes every slot at compile time.
+template <int KM>
+__device__ __forceinline__ void zwalk_interior(ZWalk& st, double (*ring)[ZLINES + 1], int lane) {
+#pragma unroll
+  for (int t = 0; t < ZCH; ++t) {
+    const int sl = (10 + ZCH * KM + t) & (ZRING - 1);
+    const double xn = ring[(sl + 9) & (ZRING - 1)][lane];
+    const double w = tg_interior(st);
+    ring[sl][lane] = w;
+    zwalk_advance(st, w, xn);
+  }
+}
+
 __global__ __launch_bounds__(ZLINES) void lad_zfilter_kernel(LadArgs a, int seg) {
   __shared__ double ring[ZRING][ZLINES + 1];
-  __shared__ uint8_t code[ZRING][ZLINES];
+  __shared__ uint8_t code[ZLINES][ZRING + 4];  // [line][slot]: row codes, read only off the fast path
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
@@ -217,10 +252,16 @@ __global__ __launch_bounds__(ZLINES) void lad_zfilter_kernel(LadArgs a, int seg)
   const int c16 = lane & (ZCH - 1), l4 = lane / ZCH;  // chunk transfers: 4 lines x 16 columns per instruction
   const int ib = max(0, S0 - ZWARM);                   // first point the walk filters
   const int P0 = ib - 10;                              // column of ring slot 0: the farthest a closure row reads back
-  // one chunk of columns [q, q+ZCH) of the 64 lines: loads into registers, then into the ring
-  double v[ZLINES / 4];
-  uint32_t cz[ZLINES / 4];
-  auto load_chunk = [&](int q) {
+  // ring chunk m holds columns P0 + 16 m .. P0 + 16 m + 15; walk chunk k (points ib + 16 k ..)
+  // spans ring chunks k and k + 1.  Loads go into registers, two sets alternating: the loads of
+  // chunk k + 4 are in flight while chunk k + 3 waits to be stored.
+  struct Chunk {
+    double v[ZLINES / 4];
+    uint32_t cz[ZLINES / 4];
+  };
+  Chunk ca, cb;
+  unsigned long long interior = 0ull;  // bit (m mod 64): every code of ring chunk m is interior
+  auto load_chunk = [&](Chunk& c, int q) {
 #pragma unroll
     for (int r = 0; r < ZLINES / 4; ++r) {
       const int j = jb + 4 * r + l4, i = q + c16;
@@ -228,91 +269,91 @@ __global__ __launch_bounds__(ZLINES) void lad_zfilter_kernel(LadArgs a, int seg)
       const long long gi = in ? (long long)i + (long long)pitch * j : 0;
       const double xv = x[gi];
       const uint32_t m = a.meta[gi];
-      v[r] = in ? xv : 0.0;
-      cz[r] = in ? (m & 15u) : ROW_BLANK;
+      c.v[r] = in ? xv : 0.0;
+      c.cz[r] = in ? (m & 15u) : ROW_BLANK;
     }
   };
-  auto store_chunk = [&](int q) {
-    const int slot = (q + c16 - P0) & (ZRING - 1);
+  auto store_chunk = [&](const Chunk& c, int m) {
+    const int slot = (m * ZCH + c16) & (ZRING - 1);
+    bool all = true;
 #pragma unroll
     for (int r = 0; r < ZLINES / 4; ++r) {
-      ring[slot][4 * r + l4] = v[r];
-      code[slot][4 * r + l4] = (uint8_t)cz[r];
+      ring[slot][4 * r + l4] = c.v[r];
+      code[4 * r + l4][slot] = (uint8_t)c.cz[r];
+      all = all && c.cz[r] == ROW_INTERIOR;
     }
+    const unsigned long long bit = 1ull << (m & 63);
+    interior = __all(all) ? (interior | bit) : (interior & ~bit);
   };
-  for (int q = P0; q < P0 + 3 * ZCH; q += ZCH) {
-    load_chunk(q);
-    store_chunk(q);
-  }
+  load_chunk(ca, P0);
+  load_chunk(cb, P0 + ZCH);
+  store_chunk(ca, 0);
+  load_chunk(ca, P0 + 2 * ZCH);
+  store_chunk(cb, 1);
+  load_chunk(cb, P0 + 3 * ZCH);  // chunk 3, stored at the end of walk chunk 0
+  store_chunk(ca, 2);
   __syncthreads();
-  int i = ib;
-  int sl = (i - P0) & (ZRING - 1);
-  // filtered values at i-1..i-4 (zeros before the line start, else the unfiltered values the
-  // warm-up forgets), unfiltered values at i..i+8 and row codes at i..i+4: each LDS read is
-  // issued several steps before its value is needed
-  double y1 = ring[(sl - 1) & (ZRING - 1)][lane], y2 = ring[(sl - 2) & (ZRING - 1)][lane];
-  double y3 = ring[(sl - 3) & (ZRING - 1)][lane], y4 = ring[(sl - 4) & (ZRING - 1)][lane];
-  double X[9];
-  uint32_t Z[5];
+  ZWalk st;
+  {
+    const int sl = 10;  // slot of ib
+    st.y1 = ring[sl - 1][lane];
+    st.y2 = ring[sl - 2][lane];
+    st.y3 = ring[sl - 3][lane];
+    st.y4 = ring[sl - 4][lane];
 #pragma unroll
-  for (int m = 0; m < 9; ++m) X[m] = ring[(sl + m) & (ZRING - 1)][lane];
+    for (int m = 0; m < 9; ++m) st.X[m] = ring[sl + m][lane];
+  }
+  // general step: closure rows, blanked points and the interior filter chosen per point
+  auto general_step = [&](int i) {
+    const int sl = (i - P0) & (ZRING - 1);
+    const double xn = ring[(sl + 9) & (ZRING - 1)][lane];
+    const uint32_t zr = code[lane][sl];
+    double w = tg_interior(st);
+    if (zr != ROW_INTERIOR) {
+      if (zr == ROW_BLANK) {
+        w = st.X[0];  // blanked points keep their value (ModTGfilter.f90:17)
+      } else {
+        // closure row: 11 ring columns from base = -row (rows 0-3) or -row-2 (rows 5-8)
+        const int base = zr < 4 ? -(int)zr : -(int)zr - 2;
+        double acc = 0.0;
 #pragma unroll
-  for (int m = 0; m < 5; ++m) Z[m] = code[(sl + m) & (ZRING - 1)][lane];
-  for (int k = 0; ib + k * ZCH < S1; ++k) {
-    const int cb = ib + k * ZCH;  // first point of this chunk
-    const int q = P0 + 3 * ZCH + k * ZCH;
-    load_chunk(q);                // two chunks ahead of the walk
-    const int n = min(ZCH, S1 - cb);
-    auto step = [&]() {
-      sl = (i - P0) & (ZRING - 1);
-      const double xn = ring[(sl + 9) & (ZRING - 1)][lane];
-      const uint32_t zn = code[(sl + 5) & (ZRING - 1)][lane];
-      const uint32_t zr = Z[0];
-      // the interior filter always; closure rows and blanked points (a few per line) replace it
-      double w = (((((((TG0 * y4 + TG1 * y3) + TG2 * y2) + TG3 * y1) + TG4 * X[0]) + TG3 * X[1]) + TG2 * X[2]) +
-                  TG1 * X[3]) + TG0 * X[4];
-      if (zr != ROW_INTERIOR) {
-        if (zr == ROW_BLANK) {
-          w = X[0];  // blanked points keep their value (ModTGfilter.f90:17)
-        } else {
-          // closure row: 11 ring columns from base = -row (rows 0-3) or -row-2 (rows 5-8)
-          const int base = zr < 4 ? -(int)zr : -(int)zr - 2;
-          double acc = 0.0;
-#pragma unroll
-          for (int m = 0; m < 11; ++m) acc = acc + c_tg[zr][m] * ring[(sl + base + m) & (ZRING - 1)][lane];
-          w = acc;
-        }
+        for (int m = 0; m < 11; ++m) acc = acc + c_tg[zr][m] * ring[(sl + base + m) & (ZRING - 1)][lane];
+        w = acc;
       }
-      ring[sl][lane] = w;
-      y4 = y3;
-      y3 = y2;
-      y2 = y1;
-      y1 = w;
-#pragma unroll
-      for (int m = 0; m < 8; ++m) X[m] = X[m + 1];
-      X[8] = xn;
-#pragma unroll
-      for (int m = 0; m < 4; ++m) Z[m] = Z[m + 1];
-      Z[4] = zn;
-      ++i;
-    };
-    if (n == ZCH) {
-      // a whole chunk, unrolled: the register queues rotate by renaming
-#pragma unroll
-      for (int t = 0; t < ZCH; ++t) step();
+    }
+    ring[sl][lane] = w;
+    zwalk_advance(st, w, xn);
+  };
+  auto walk_chunk = [&](int k, Chunk& next, const Chunk& ready) {
+    const int c0 = ib + k * ZCH;  // first point of this chunk
+    load_chunk(next, P0 + (k + 4) * ZCH);
+    const int n = min(ZCH, S1 - c0);
+    const bool fast = n == ZCH && ((interior >> (k & 63)) & 1ull) && ((interior >> ((k + 1) & 63)) & 1ull);
+    if (fast) {
+      switch (k & 3) {
+        case 0: zwalk_interior<0>(st, ring, lane); break;
+        case 1: zwalk_interior<1>(st, ring, lane); break;
+        case 2: zwalk_interior<2>(st, ring, lane); break;
+        default: zwalk_interior<3>(st, ring, lane); break;
+      }
     } else {
-      for (int t = 0; t < n; ++t) step();
+      for (int t = 0; t < n; ++t) general_step(c0 + t);
     }
     __syncthreads();
     // write back the chunk's points that belong to the segment
 #pragma unroll
     for (int r = 0; r < ZLINES / 4; ++r) {
-      const int j = jb + 4 * r + l4, ii = cb + c16;
+      const int j = jb + 4 * r + l4, ii = c0 + c16;
       if (j < nr && ii >= S0 && ii < S1 && c16 < n)
         y[(long long)ii + (long long)pitch * j] = ring[(ii - P0) & (ZRING - 1)][4 * r + l4];
     }
-    store_chunk(q);  // overwrites the columns 64 behind q, already written back
+    store_chunk(ready, k + 3);  // overwrites the columns 64 behind, already written back
     __syncthreads();
+  };
+  const int nchunks = (S1 - ib + ZCH - 1) / ZCH;
+  for (int k = 0; k < nchunks; k += 2) {
+    walk_chunk(k, ca, cb);
+    if (k + 1 < nchunks) walk_chunk(k + 1, cb, ca);
   }
 }
 
