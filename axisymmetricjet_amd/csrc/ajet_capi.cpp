@@ -34,6 +34,8 @@ struct ajet_ctx {
   int n_int = 0, n_bnd = 0;
   int *lad_int = nullptr, *lad_bnd = nullptr;      // the same split for the LAD kernels
   int n_lad_int = 0, n_lad_bnd = 0;
+  int* lad_groups = nullptr;                       // line groups of the fused z/r filter
+  int n_lad_groups = 0;
   StepScalars* sc = nullptr;
   double* dtlog = nullptr;
   long long logcap = 1 << 20;
@@ -108,7 +110,7 @@ int ajet_destroy(ajet_ctx* c) {
   void* bufs[] = {c->Q0, c->Q1, c->Q2, c->A, c->S, c->tgt, c->lad, c->ladA, c->ladB, c->dzl, c->drl, c->ramp,
                   c->meta, c->zc, c->rc, c->dxidz,
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
-                  c->int_tiles, c->bnd_tiles, c->lad_int, c->lad_bnd, c->sc, c->dtlog};
+                  c->int_tiles, c->bnd_tiles, c->lad_int, c->lad_bnd, c->lad_groups, c->sc, c->dtlog};
   if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -231,6 +233,11 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
       HIPCHK(c, hipMemcpy(c->lad_int, it.data(), sizeof(int) * it.size(), hipMemcpyHostToDevice));
     if (!bt.empty())
       HIPCHK(c, hipMemcpy(c->lad_bnd, bt.data(), sizeof(int) * bt.size(), hipMemcpyHostToDevice));
+    std::vector<int> gr;
+    build_lad_line_groups(c->s, nz, nr, LAD_LINES, gr);
+    c->n_lad_groups = (int)gr.size() / 3;
+    HIPCHK(c, hipMalloc((void**)&c->lad_groups, sizeof(int) * gr.size()));
+    HIPCHK(c, hipMemcpy(c->lad_groups, gr.data(), sizeof(int) * gr.size(), hipMemcpyHostToDevice));
   }
   HIPCHK(c, hipMalloc((void**)&c->sc, sizeof(StepScalars)));
   HIPCHK(c, hipMemsetAsync(c->sc, 0, sizeof(StepScalars), c->stream));
@@ -325,6 +332,8 @@ static LadArgs lad_args(ajet_ctx* c, const double* q) {
   l.bnd_tiles = c->lad_bnd;
   l.n_int = c->n_lad_int;
   l.n_bnd = c->n_lad_bnd;
+  l.rgroups = c->lad_groups;
+  l.nrgroups = c->n_lad_groups;
   l.g = c->g;
   l.gam = c->p.gam;
   // ModShockCapturing.f90:66, 72, 79: C_mu Re, C_beta Re, C_kappa Re Pr

@@ -28,6 +28,8 @@ constexpr int WZ = TZ + 2 * HALO;
 constexpr int WR = TR + 2 * HALO;
 constexpr int STAGE_THREADS = AJET_THREADS;
 
+constexpr int LAD_LINES = 64;  // r-lines one wave of the fused z/r filter walks
+
 constexpr int MAX_PATCH = 64;
 constexpr int MAX_SEQ = 64;       // distinct ordered lists of boundary patches met at one point
 constexpr int SEQ_LEN = 6;
@@ -85,6 +87,7 @@ int build_setup(const ajet_params& p, const double* z, const double* r, const in
 int build_tiling(Setup& s, int nz, int nr, std::string& err);
 void classify_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, std::vector<int>& boundary);
 void classify_lad_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, std::vector<int>& boundary);
+void build_lad_line_groups(const Setup& s, int nz, int nr, int lines, std::vector<int>& groups);
 void lad_tables(const Setup& s, int nz, int nr, std::vector<double>& dz, std::vector<double>& dr,
                 std::vector<double>& ramp);
 

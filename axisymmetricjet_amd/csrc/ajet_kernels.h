@@ -61,7 +61,7 @@ struct LadArgs {
   const double* dzl;       // grid spacing of ComputeTobeFilteredQuantities along z (nz)
   const double* drl;       // ... along r (nr)
   const double* ramp;      // (1 + tanh(5 (z - 10))) / 2 (nz)
-  double* A;               // 3 fields: S, dil, f_sw; later the z-filtered quantities
+  double* A;               // 3 fields: S, dil, f_sw
   double* B;               // 3 fields: the quantities before filtering
   double* out;             // 3 fields: mu*, beta*, kappa*
   const int* zcuts;
@@ -69,6 +69,8 @@ struct LadArgs {
   const int* int_tiles;    // tiles whose points and stencil rows are all interior
   const int* bnd_tiles;    // the others
   int n_int, n_bnd;
+  const int* rgroups;      // line groups of the fused z/r filter: (wb, out_lo, out_hi) each
+  int nrgroups;
   Geometry g;
   double gam;
   double cmu, cbeta, ckappa;  // C_mu Re, C_beta Re, C_kappa Re Pr

@@ -3,9 +3,9 @@
 //
 //   lad_strain_kernel   |S|, dilatation and the Ducros-type switch f_sw at every point
 //   lad_source_kernel   D2(D2(.)) of |S|, dil and T in xi and in eta; the three quantities
-//   lad_zfilter_kernel  the z filter, applied in place by the reference, i.e. the causal
-//                       recursion y_i = sum_{m<0} c_m y_{i+m} + sum_{m>=0} c_m x_{i+m}
-//   lad_rfilter_kernel  the r filter, C Re (Pr) scaling and the tanh ramp -> mu*, beta*, kappa*
+//   lad_zrfilter_kernel the z filter, applied in place by the reference, i.e. the causal
+//                       recursion y_i = sum_{m<0} c_m y_{i+m} + sum_{m>=0} c_m x_{i+m}, then
+//                       the r filter, C Re (Pr) scaling and the tanh ramp -> mu*, beta*, kappa*
 //
 // Build with -ffp-contract=off.
 #include <hip/hip_runtime.h>
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* _
 // from ib - 10 + 16 (k - 1) (ib - 10 at k = 0) up to ib + 37 + 16 k, which covers every read
 // of the chunk's steps i = ib + 16 k .. ib + 16 k + 15 (columns i - 10 .. i + 10).
 constexpr int ZWARM = 48;
-constexpr int ZLINES = 64;
+constexpr int ZLINES = LAD_LINES;
 constexpr int ZCH = 16;
 constexpr int ZRING = 64;
 
@@ -236,18 +236,26 @@ __device__ __forceinline__ void zwalk_interior(ZWalk& st, double (*ring)[ZLINES 
   }
 }
 
-__global__ __launch_bounds__(ZLINES) void lad_zfilter_kernel(LadArgs a, int seg) {
+// The fused z and r filters.  Block (segment, line group, field): one wave walks the group's 64
+// r-lines wb..wb+63 through the z recursion, then, chunk by chunk, takes the r filter of the
+// group's output lines [out_lo, out_hi) from the ring (PerformTGfilter along eta,
+// ModShockCapturing.f90:64, 70, 76), scales it by C Re (C_kappa Re Pr) and the ramp
+// (ModShockCapturing.f90:66-92) and writes mu*, beta* or kappa*.  Blanked points are left out
+// by the filter and keep zero.  Ring codes: z row in the low nibble, r row in the high one.
+__global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg) {
   __shared__ double ring[ZRING][ZLINES + 1];
-  __shared__ uint8_t code[ZLINES][ZRING + 4];  // [line][slot]: row codes, read only off the fast path
+  __shared__ uint8_t code[ZLINES][ZRING + 4];  // [line][slot]
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
   const int S0 = (int)blockIdx.x * seg;
   const int S1 = min(nz, S0 + seg);
-  const int jb = (int)blockIdx.y * ZLINES;
+  const int jb = a.rgroups[3 * blockIdx.y];
+  const int out_lo = a.rgroups[3 * blockIdx.y + 1], out_hi = a.rgroups[3 * blockIdx.y + 2];
   const int f = (int)blockIdx.z;
   const double* __restrict__ x = a.B + f * fs;
-  double* __restrict__ y = a.A + f * fs;
+  double* __restrict__ out = a.out + f * fs;
+  const double C = f == 0 ? a.cmu : (f == 1 ? a.cbeta : a.ckappa);
   const int lane = threadIdx.x;
   const int c16 = lane & (ZCH - 1), l4 = lane / ZCH;  // chunk transfers: 4 lines x 16 columns per instruction
   const int ib = max(0, S0 - ZWARM);                   // first point the walk filters
@@ -260,7 +268,7 @@ __global__ __launch_bounds__(ZLINES) void lad_zfilter_kernel(LadArgs a, int seg)
     uint32_t cz[ZLINES / 4];
   };
   Chunk ca, cb;
-  unsigned long long interior = 0ull;  // bit (m mod 64): every code of ring chunk m is interior
+  unsigned long long interior = 0ull;  // bit (m mod 64): every z code of ring chunk m is interior
   auto load_chunk = [&](Chunk& c, int q) {
 #pragma unroll
     for (int r = 0; r < ZLINES / 4; ++r) {
@@ -270,7 +278,7 @@ __global__ __launch_bounds__(ZLINES) void lad_zfilter_kernel(LadArgs a, int seg)
       const double xv = x[gi];
       const uint32_t m = a.meta[gi];
       c.v[r] = in ? xv : 0.0;
-      c.cz[r] = in ? (m & 15u) : ROW_BLANK;
+      c.cz[r] = in ? (m & 255u) : (ROW_BLANK | (ROW_BLANK << 4));
     }
   };
   auto store_chunk = [&](const Chunk& c, int m) {
@@ -280,7 +288,7 @@ __global__ __launch_bounds__(ZLINES) void lad_zfilter_kernel(LadArgs a, int seg)
     for (int r = 0; r < ZLINES / 4; ++r) {
       ring[slot][4 * r + l4] = c.v[r];
       code[4 * r + l4][slot] = (uint8_t)c.cz[r];
-      all = all && c.cz[r] == ROW_INTERIOR;
+      all = all && (c.cz[r] & 15u) == ROW_INTERIOR;
     }
     const unsigned long long bit = 1ull << (m & 63);
     interior = __all(all) ? (interior | bit) : (interior & ~bit);
@@ -307,7 +315,7 @@ __global__ __launch_bounds__(ZLINES) void lad_zfilter_kernel(LadArgs a, int seg)
   auto general_step = [&](int i) {
     const int sl = (i - P0) & (ZRING - 1);
     const double xn = ring[(sl + 9) & (ZRING - 1)][lane];
-    const uint32_t zr = code[lane][sl];
+    const uint32_t zr = code[lane][sl] & 15u;
     double w = tg_interior(st);
     if (zr != ROW_INTERIOR) {
       if (zr == ROW_BLANK) {
@@ -323,6 +331,46 @@ __global__ __launch_bounds__(ZLINES) void lad_zfilter_kernel(LadArgs a, int seg)
     }
     ring[sl][lane] = w;
     zwalk_advance(st, w, xn);
+  };
+  // the r filter of one column: lane (c16, l4) takes lines out_lo + l4 * per .. of column c16,
+  // interior rows from a sliding window of 9 lines
+  const int nout = out_hi - out_lo;
+  const int per = (nout + 3) / 4;
+  auto rfilter_chunk = [&](int c0, int n) {
+    const int ii = c0 + c16;
+    if (c16 >= n || ii < S0 || ii >= S1) return;
+    const int sl = (ii - P0) & (ZRING - 1);
+    const double ramp = a.ramp[ii];
+    const int j0 = out_lo + l4 * per;
+    const int j1 = min(out_hi, j0 + per);
+    auto val = [&](int j) {  // z-filtered value of line j (zero outside the grid)
+      const int l = j - jb;
+      return (j >= 0 && j < nr && l >= 0 && l < ZLINES) ? ring[sl][l] : 0.0;
+    };
+    double w[9];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) w[m + 1] = val(j0 - 4 + m);
+    for (int j = j0; j < j1; ++j) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) w[m] = w[m + 1];
+      w[8] = val(j + 4);
+      const int l = j - jb;
+      const uint32_t rr = code[l][sl] >> 4;
+      double v = 0.0;
+      if (rr == ROW_INTERIOR) {
+        v = (((((((TG0 * w[0] + TG1 * w[1]) + TG2 * w[2]) + TG3 * w[3]) + TG4 * w[4]) + TG3 * w[5]) + TG2 * w[6]) +
+             TG1 * w[7]) + TG0 * w[8];
+      } else if (rr != ROW_BLANK) {
+        const int base = rr < 4 ? -(int)rr : -(int)rr - 2;
+#pragma unroll
+        for (int m = 0; m < 11; ++m) v = v + c_tg[rr][m] * val(j + base + m);
+      }
+      if (rr != ROW_BLANK) {
+        v = C * v;
+        v = v * ramp;
+      }
+      out[(long long)ii + (long long)pitch * j] = v;
+    }
   };
   auto walk_chunk = [&](int k, Chunk& next, const Chunk& ready) {
     const int c0 = ib + k * ZCH;  // first point of this chunk
@@ -340,80 +388,14 @@ __global__ __launch_bounds__(ZLINES) void lad_zfilter_kernel(LadArgs a, int seg)
       for (int t = 0; t < n; ++t) general_step(c0 + t);
     }
     __syncthreads();
-    // write back the chunk's points that belong to the segment
-#pragma unroll
-    for (int r = 0; r < ZLINES / 4; ++r) {
-      const int j = jb + 4 * r + l4, ii = c0 + c16;
-      if (j < nr && ii >= S0 && ii < S1 && c16 < n)
-        y[(long long)ii + (long long)pitch * j] = ring[(ii - P0) & (ZRING - 1)][4 * r + l4];
-    }
-    store_chunk(ready, k + 3);  // overwrites the columns 64 behind, already written back
+    rfilter_chunk(c0, n);
+    store_chunk(ready, k + 3);  // overwrites the columns 64 behind, already filtered in r
     __syncthreads();
   };
   const int nchunks = (S1 - ib + ZCH - 1) / ZCH;
   for (int k = 0; k < nchunks; k += 2) {
     walk_chunk(k, ca, cb);
     if (k + 1 < nchunks) walk_chunk(k + 1, cb, ca);
-  }
-}
-
-// ---- the r filter (PerformTGfilter along eta into mu*, beta*, kappa*), then the scaling by
-// C Re (C_kappa Re Pr) and the ramp (ModShockCapturing.f90:64-92).  Blanked points are left
-// out by the filter and keep zero.  Each thread walks RROWS rows of one column with the
-// interior window in registers; closure rows read their 11 points directly.
-constexpr int RROWS = 8;
-
-__global__ __launch_bounds__(256) void lad_rfilter_kernel(LadArgs a) {
-  const Geometry& g = a.g;
-  const int nz = g.nz, nr = g.nr, pitch = g.pitch;
-  const long long fs = g.fstride;
-  const int i = (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
-  const int jr = ((int)blockIdx.y * 4 + (int)(threadIdx.x >> 6)) * RROWS;
-  if (i >= nz || jr >= nr) return;
-  const double ramp = a.ramp[i];
-  uint32_t rr[RROWS];
-#pragma unroll
-  for (int m = 0; m < RROWS; ++m) {
-    const bool in = jr + m < nr;
-    const uint32_t mw = a.meta[(long long)i + (long long)pitch * (in ? jr + m : 0)];
-    rr[m] = in ? (mw >> 4) & 15u : ROW_BLANK;
-  }
-  const double C[3] = {a.cmu, a.cbeta, a.ckappa};
-#pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    const double* col = a.A + f * fs + i;
-    double w[RROWS + 8];  // rows jr-4 .. jr+RROWS+3, zero outside the grid
-#pragma unroll
-    for (int m = 0; m < RROWS + 8; ++m) {
-      const int jj = jr - 4 + m;
-      const bool in = jj >= 0 && jj < nr;
-      const double wv = col[(long long)pitch * (in ? jj : 0)];
-      w[m] = in ? wv : 0.0;
-    }
-#pragma unroll
-    for (int m = 0; m < RROWS; ++m) {
-      const int j = jr + m;
-      if (j >= nr) break;
-      double v = 0.0;
-      if (rr[m] == ROW_INTERIOR) {
-        v = (((((((TG0 * w[m] + TG1 * w[m + 1]) + TG2 * w[m + 2]) + TG3 * w[m + 3]) + TG4 * w[m + 4]) +
-               TG3 * w[m + 5]) + TG2 * w[m + 6]) + TG1 * w[m + 7]) + TG0 * w[m + 8];
-        v = C[f] * v;
-        v = v * ramp;
-      } else if (rr[m] != ROW_BLANK) {
-        const int lo = rr[m] < 4 ? j - (int)rr[m] : j - (int)rr[m] - 2;
-        v = 0.0;
-        for (int t = 0; t < 11; ++t) {
-          const int jj = lo + t;
-          const bool in = jj >= 0 && jj < nr;
-          const double xv = col[(long long)pitch * (in ? jj : 0)];
-          v = v + c_tg[rr[m]][t] * (in ? xv : 0.0);
-        }
-        v = C[f] * v;
-        v = v * ramp;
-      }
-      a.out[(long long)i + (long long)pitch * j + f * fs] = v;
-    }
   }
 }
 
@@ -432,10 +414,7 @@ hipError_t launch_lad(const LadArgs& a, hipStream_t s) {
   if (a.n_int > 0) hipLaunchKernelGGL(lad_source_kernel<true>, grid(a.n_int), dim3(256), 0, s, a, a.int_tiles, a.n_int);
   if (a.n_bnd > 0) hipLaunchKernelGGL(lad_source_kernel<false>, grid(a.n_bnd), dim3(256), 0, s, a, a.bnd_tiles, a.n_bnd);
   const int seg = zfilter_segment(g.nz);
-  hipLaunchKernelGGL(lad_zfilter_kernel, dim3((g.nz + seg - 1) / seg, (g.nr + ZLINES - 1) / ZLINES, 3), dim3(ZLINES),
-                     0, s, a, seg);
-  hipLaunchKernelGGL(lad_rfilter_kernel, dim3((g.nz + 63) / 64, (g.nr + 4 * RROWS - 1) / (4 * RROWS)), dim3(256), 0,
-                     s, a);
+  hipLaunchKernelGGL(lad_zrfilter_kernel, dim3((g.nz + seg - 1) / seg, a.nrgroups, 3), dim3(ZLINES), 0, s, a, seg);
   return hipGetLastError();
 }
 

@@ -307,6 +307,45 @@ void classify_lad_tiles(const Setup& s, int nz, int nr, std::vector<int>& interi
     }
 }
 
+// Line groups of the fused z/r filter: each group walks `lines` consecutive r-lines from wb
+// through the z recursion and writes the r-filtered output of lines [out_lo, out_hi).  The r
+// filter of line j reads lines j-4..j+4 (interior rows) or its closure row's 11 lines, clipped
+// to the grid; a group holds every line those windows touch.  Groups are triples (wb, out_lo,
+// out_hi); the last ones are pulled down so that all their walked lines are inside the grid.
+void build_lad_line_groups(const Setup& s, int nz, int nr, int lines, std::vector<int>& groups) {
+  std::vector<int> lo(nr), hi(nr);
+  for (int j = 0; j < nr; ++j) {
+    lo[j] = j;
+    hi[j] = j;
+    for (int i = 0; i < nz; ++i) {
+      const int rr = (int)((s.meta[(size_t)i + (size_t)nz * j] >> 4) & 15u);
+      int b, e;
+      if (rr == (int)ROW_BLANK) continue;
+      if (rr == (int)ROW_INTERIOR) { b = j - 4; e = j + 4; }
+      else { b = rr < 4 ? j - rr : j - rr - 2; e = b + 10; }
+      lo[j] = std::min(lo[j], std::max(b, 0));
+      hi[j] = std::max(hi[j], std::min(e, nr - 1));
+    }
+  }
+  groups.clear();
+  int j = 0;
+  while (j < nr) {
+    int mlo = lo[j], mhi = hi[j];
+    int k = j + 1;
+    while (k < nr && std::max(mhi, hi[k]) - std::min(mlo, lo[k]) + 1 <= lines) {
+      mlo = std::min(mlo, lo[k]);
+      mhi = std::max(mhi, hi[k]);
+      ++k;
+    }
+    int wb = mlo;
+    if (wb + lines > nr) wb = std::max(0, std::min(mlo, nr - lines));
+    groups.push_back(wb);
+    groups.push_back(j);
+    groups.push_back(k);
+    j = k;
+  }
+}
+
 int build_tiling(Setup& s, int nz, int nr, std::string& err) {
   const int BIG = 1 << 30;
   std::vector<int> zlo(nz, BIG), zhi(nz, -BIG), rlo(nr, BIG), rhi(nr, -BIG);
