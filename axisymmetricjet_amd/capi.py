@@ -18,7 +18,7 @@ ERRORS = {-1: "EINVAL", -2: "EGRID", -3: "EPATCH", -4: "ENODEV", -5: "EBLOWUP", 
 
 EXPORTS = ["ajet_params_default", "ajet_build_patches", "ajet_create", "ajet_destroy", "ajet_last_error",
            "ajet_set_state", "ajet_get_state", "ajet_compute_dt", "ajet_rk4_stage", "ajet_rk4_step", "ajet_run",
-           "ajet_get_dt_log", "ajet_rhs", "ajet_lad_coeffs", "ajet_stream", "ajet_version"]
+           "ajet_get_dt_log", "ajet_rhs", "ajet_lad_coeffs", "ajet_set_forcing", "ajet_stream", "ajet_version"]
 
 
 class Params(C.Structure):
@@ -71,11 +71,17 @@ def lib():
         L.ajet_get_dt_log.argtypes = [vp, vp, C.c_int32, vp]
         L.ajet_rhs.argtypes = [vp, dp, dp]
         L.ajet_lad_coeffs.argtypes = [vp, dp, dp, dp, dp]
+        L.ajet_set_forcing.argtypes = [vp, C.POINTER(ForcingC)]
         L.ajet_stream.argtypes = [vp]
         L.ajet_stream.restype = vp
         L.ajet_version.restype = C.c_char_p
         _lib = L
     return _lib
+
+
+class ForcingC(C.Structure):
+    _fields_ = [("F", C.c_double * 25)] + [(n, C.c_double) for n in
+                                           ("z0", "r0", "lz", "lr", "amp", "strouhal", "uj", "t0")]
 
 
 def default_params(nz, nr, **kw) -> Params:
@@ -110,7 +116,7 @@ class Context:
     """One device context: the time-step state of one jet run on the current GPU."""
 
     def __init__(self, case=None, *, params: Params | None = None, z=None, r=None, iblank=None, bc_lines=None,
-                 **pkw):
+                 forcing=None, **pkw):
         L = lib()
         if case is not None:
             z, r, iblank, bc_lines = case.z, case.r, case.iblank, case.bc_lines
@@ -119,6 +125,8 @@ class Context:
         z, r = F(z), F(r)
         nz, nr = z.shape
         self.nz, self.nr = nz, nr
+        if forcing is not None:
+            pkw.setdefault("forcing", int(forcing.mode))
         self.params = params or default_params(nz, nr, **pkw)
         self._ib = None if iblank is None else np.asfortranarray(iblank, dtype=np.int32)
         bcs, n = bc_array(bc_lines or [])
@@ -132,6 +140,8 @@ class Context:
                 L.ajet_destroy(h)
             self.h = None
             raise AjetError(rc, msg)
+        if forcing is not None:
+            self.set_forcing(forcing)
         if case is not None:
             self.set_state(case.q0, case.target)
 
@@ -188,6 +198,14 @@ class Context:
         out = np.zeros((self.nz, self.nr, 5), order="F")
         self._check(lib().ajet_rhs(self.h, F(q), out), "ajet_rhs")
         return out
+
+    def set_forcing(self, f):
+        fc = ForcingC()
+        for k, v in enumerate(np.asarray(f.F, dtype=np.float64).ravel()):
+            fc.F[k] = v
+        for n in ("z0", "r0", "lz", "lr", "amp", "strouhal", "uj", "t0"):
+            setattr(fc, n, float(getattr(f, n)))
+        self._check(lib().ajet_set_forcing(self.h, C.byref(fc)), "ajet_set_forcing")
 
     def lad_coeffs(self, q):
         """mu*, beta*, kappa* of ComputeArtTransCoeffs for the state q, each (nz, nr)."""

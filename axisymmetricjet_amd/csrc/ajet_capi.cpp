@@ -43,6 +43,9 @@ struct ajet_ctx {
   bool fused_valid = false;  // sc->dmax_bits holds the CFL reduction of the current state
   bool pending_finalize = false;  // the next stage 1 turns that reduction into dt
   bool have_state = false;
+  bool forcing_set = false;  // ajet_set_forcing was called
+  int parity = 0;            // step parity (which StepScalars.tpar slot holds the step start time)
+  ForcingDev fdev{};
   std::string err;
 };
 
@@ -131,7 +134,8 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   c->p = *p;
   int rc = build_setup(*p, z, r, iblank, lines, nlines, c->s, c->err);
   if (rc == AJET_OK) rc = build_tiling(c->s, p->nz, p->nr, c->err);
-  if (rc == AJET_OK && p->forcing != 0) rc = fail(c, AJET_EUNSUPPORTED, "inflow forcing is not available");
+  if (rc == AJET_OK && (p->forcing & ~(AJET_FORCING_FEEDBACK | AJET_FORCING_EXCITATION)) != 0)
+    rc = fail(c, AJET_EINVAL, "unknown forcing bits");
   int ndev = 0;
   if (rc == AJET_OK && (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1))
     rc = fail(c, AJET_ENODEV, "no HIP device");
@@ -268,6 +272,27 @@ int ajet_set_state(ajet_ctx* c, const double* pv, const double* pv_target) {
   c->fused_valid = false;
   c->pending_finalize = false;
   c->have_state = true;
+  c->parity = 0;
+  return AJET_OK;
+}
+
+int ajet_set_forcing(ajet_ctx* c, const ajet_forcing* f) {
+  if (!c || !f) return AJET_EINVAL;
+  if (c->p.forcing == 0) return fail(c, AJET_EINVAL, "the context was created without forcing");
+  if (!(f->lz > 0.0) || !(f->lr > 0.0)) return fail(c, AJET_EINVAL, "forcing widths lz, lr must be positive");
+  ForcingDev d{};
+  const double alpha = 6.0;  // ModuleVariables.f90:184
+  for (int e = 0; e < 5; ++e)
+    for (int k = 0; k < 5; ++k) d.aF[e][k] = alpha * f->F[e][k];
+  d.z0 = f->z0;
+  d.r0 = f->r0;
+  d.lz2 = f->lz * f->lz;
+  d.lr2 = f->lr * f->lr;
+  d.au = f->amp * f->uj;
+  d.w = 2.0 * 3.14159265358979323846 * f->strouhal * f->uj;
+  d.t0 = f->t0;
+  c->fdev = d;
+  c->forcing_set = true;
   return AJET_OK;
 }
 
@@ -302,6 +327,9 @@ static StageArgs stage_args(ajet_ctx* c) {
   a.logcap = c->logcap;
   a.CFL = c->p.CFL;
   a.dt_from_dmax = 0;
+  a.forcing = c->p.forcing;
+  a.parity = c->parity;
+  a.fc = c->fdev;
   return a;
 }
 
@@ -411,6 +439,7 @@ int ajet_compute_dt(ajet_ctx* c, double* dt) {
 int ajet_rk4_stage(ajet_ctx* c, int32_t stage) {
   if (!c || !c->stream || stage < 1 || stage > 4) return AJET_EINVAL;
   if (!c->dt_valid) return fail(c, AJET_EINVAL, "no time step: call ajet_compute_dt first");
+  if (c->p.forcing && !c->forcing_set) return fail(c, AJET_EINVAL, "forcing parameters not set: call ajet_set_forcing");
   StageArgs a = stage_args(c);
   a.qn = c->Q0;
   a.acc_a = c->A;
@@ -425,6 +454,7 @@ int ajet_rk4_stage(ajet_ctx* c, int32_t stage) {
   HIPCHK(c, launch_stages(c, stage, a));
   if (stage == 1) c->pending_finalize = false;
   c->fused_valid = stage == 4;
+  if (stage == 4) c->parity ^= 1;
   return AJET_OK;
 }
 
@@ -467,6 +497,7 @@ int ajet_get_dt_log(ajet_ctx* c, double* dts, int32_t n, double* time) {
 
 int ajet_rhs(ajet_ctx* c, const double* q, double* rhs) {
   if (!c || !c->stream || !q || !rhs) return AJET_EINVAL;
+  if (c->p.forcing && !c->forcing_set) return fail(c, AJET_EINVAL, "forcing parameters not set: call ajet_set_forcing");
   HIPCHK(c, hipSetDevice(c->device));
   copy_state_in(c, c->Q2, q);
   StageArgs a = stage_args(c);

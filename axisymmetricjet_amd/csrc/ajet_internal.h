@@ -55,6 +55,9 @@ struct StepScalars {
   long long nlog;         // entries written to the time-step log
   int32_t nonfinite;      // a non-finite denominator was met
   int32_t pad;
+  double tpar[2];         // time at the start of the step, by step parity: the step reads one
+                          // slot and its last stage writes the other, so no launch reads a
+                          // value it also writes
 };
 
 struct Geometry {

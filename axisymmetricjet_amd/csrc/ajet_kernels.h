@@ -8,6 +8,13 @@
 
 namespace ajet {
 
+// Forcing terms (ControlForcingTerms ModControlForcing.f90:6-65, and the inflow excitation).
+struct ForcingDev {
+  double aF[5][5];         // alpha * forcing_mat(e,k), alpha = 6 (ModuleVariables.f90:184)
+  double z0, r0, lz2, lr2; // Gaussian centre, lz*lz, lr*lr
+  double au, w, t0;        // excitation: amp*Uj, 2 pi St Uj, time offset
+};
+
 struct StageArgs {
   const double* qin;       // stage input state (5 fields), read with a 2-point halo
   const double* qn;        // q^n (stages 2, 3)
@@ -37,6 +44,9 @@ struct StageArgs {
   long long logcap;
   double CFL;
   int dt_from_dmax;        // stage 1: dt = CFL / (max denominator of the last stage 4)
+  int forcing;             // AJET_FORCING_* bits
+  int parity;              // step parity: the stage time is sc->tpar[parity] + c dt
+  ForcingDev fc;
 };
 
 struct CflArgs {

@@ -78,7 +78,8 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long* d
 // MODE: MODE_PLAIN (viscous), MODE_SC (viscous with the per-point coefficients of
 // ShockCapturing = 1 and zero artificial terms), MODE_LAD (with mu*, beta*, kappa*),
 // MODE_INVISCID (do_viscous_terms = 0).
-template <int STAGE, int MODE, bool INT>
+// FORCE: the forcing terms are compiled in (a.forcing != 0).
+template <int STAGE, int MODE, bool INT, bool FORCE>
 __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void stage_kernel(StageArgs a) {
   constexpr bool LAD = MODE == MODE_LAD;
   constexpr bool VISC = MODE != MODE_INVISCID;
@@ -182,12 +183,16 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
   if (STAGE == 2 && !INT && li == 0 && threadIdx.x == 0) a.sc->dmax_bits = 0ull;
   double dmax = 0.0;
   double sTst = 0.0;
+  // time of this stage's RHS (used by the inflow excitation): step start + c dt
+  const double tstep = a.sc->tpar[a.parity];
+  const double tst = STAGE <= 1 ? tstep : (STAGE == 4 ? tstep + dt : tstep + 0.5 * dt);
   if (STAGE == 4) {
     const double Tst = a.qin[(long long)g.iT + (long long)pitch * g.jT + 4 * fs];
     sTst = sqrt(Tst);
     if (!INT && li == 0 && threadIdx.x == 0) {
       a.sc->T_stale = Tst;
       a.sc->time = a.sc->time + dt;
+      a.sc->tpar[a.parity ^ 1] = tstep + dt;
     }
   }
 
@@ -225,7 +230,7 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
     const bool top = r >= a.pc.sponge_start_top;
     const bool inflow = j >= 1 && zc <= a.pc.sponge_end_left;
     double qt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (sid != 0 || top || inflow) {
+    if (sid != 0 || top || inflow || (FORCE && (a.forcing & AJET_FORCING_FEEDBACK))) {
 #pragma unroll
       for (int f = 0; f < 5; ++f) qt[f] = a.tgt[gi + f * fs];
     }
@@ -299,6 +304,20 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
         }
         viscous(q, r, mu, kappa, lambda, SC, d, a.pc, rhs);
       }
+    }
+
+    // forcing (ModRHS.f90:223-225), at every point, before the boundary conditions
+    if (FORCE) {
+      const double dzf = zc - a.fc.z0, drf = r - a.fc.r0;
+      const double G = exp(-1.0 * ((dzf * dzf) / a.fc.lz2) - (drf * drf) / a.fc.lr2);
+      if (a.forcing & AJET_FORCING_FEEDBACK) {
+        // rhs(e) += alpha*forcing_mat(e,k)*GaussianFactor*(pv(k) - pv_target(k)), k in order
+#pragma unroll
+        for (int e = 0; e < 5; ++e)
+#pragma unroll
+          for (int k = 0; k < 5; ++k) rhs[e] = rhs[e] + (a.fc.aF[e][k] * G) * (q[k] - qt[k]);
+      }
+      if (a.forcing & AJET_FORCING_EXCITATION) rhs[3] = rhs[3] + (a.fc.au * sin(a.fc.w * (a.fc.t0 + tst))) * G;
     }
 
     // boundary conditions (ModBC.f90:16-277), patches in bc-file order, then the sponges
@@ -430,15 +449,21 @@ hipError_t upload_tables(const double sbp1[9][6], const double sbp2[9][6], const
 
 static int pad8(int n) { return (n + 7) / 8 * 8; }
 
+template <int MODE, bool INT, bool FORCE>
+static void launch_stage_t(int stage, dim3 grid, dim3 block, const StageArgs& a, hipStream_t s) {
+  switch (stage) {
+    case 0: hipLaunchKernelGGL((stage_kernel<0, MODE, INT, FORCE>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((stage_kernel<1, MODE, INT, FORCE>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((stage_kernel<2, MODE, INT, FORCE>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((stage_kernel<3, MODE, INT, FORCE>), grid, block, 0, s, a); break;
+    default: hipLaunchKernelGGL((stage_kernel<4, MODE, INT, FORCE>), grid, block, 0, s, a); break;
+  }
+}
+
 template <int MODE, bool INT>
 static void launch_mode(int stage, dim3 grid, dim3 block, const StageArgs& a, hipStream_t s) {
-  switch (stage) {
-    case 0: hipLaunchKernelGGL((stage_kernel<0, MODE, INT>), grid, block, 0, s, a); break;
-    case 1: hipLaunchKernelGGL((stage_kernel<1, MODE, INT>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((stage_kernel<2, MODE, INT>), grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((stage_kernel<3, MODE, INT>), grid, block, 0, s, a); break;
-    default: hipLaunchKernelGGL((stage_kernel<4, MODE, INT>), grid, block, 0, s, a); break;
-  }
+  if (a.forcing) launch_stage_t<MODE, INT, true>(stage, grid, block, a, s);
+  else launch_stage_t<MODE, INT, false>(stage, grid, block, a, s);
 }
 
 hipError_t launch_stage(int stage, int mode, bool interior, const StageArgs& a, hipStream_t s) {

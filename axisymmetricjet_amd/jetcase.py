@@ -22,6 +22,7 @@ import struct
 import numpy as np
 
 GAMMA = 1.4
+TJ_TINF = 0.6  # jet-to-ambient temperature ratio of the profile (write_soln_file.f90:31-33)
 
 
 @dataclasses.dataclass
@@ -37,6 +38,34 @@ class JetCase:
     mach: float
     re: float
     pr: float
+
+
+@dataclasses.dataclass
+class Forcing:
+    """Forcing terms (ajet_capi.h AJET_FORCING_*): mode bits 1 = ControlForcingTerms feedback
+    alpha*F(e,k)*G*(q_k - q_target_k) (ModControlForcing.f90:46-62), 2 = inflow excitation
+    amp*Uj*sin(2 pi St Uj (t0 + t))*G on the Vz equation; G the Gaussian at (z0, r0) of widths
+    lz, lr.  F is forcing_mat, row e = equation."""
+    mode: int
+    F: np.ndarray
+    z0: float = 1.0
+    r0: float = 0.5
+    lz: float = 0.5
+    lr: float = 0.1
+    amp: float = 0.0
+    strouhal: float = 0.0
+    uj: float = 0.0
+    t0: float = 0.0
+
+
+def excitation(mach: float, strouhal: float, amp: float, z0: float = 1.0, r0: float = 0.5, lz: float = 0.5,
+               lr: float = 0.1, feedback=None) -> Forcing:
+    """Inflow excitation of the jet of Mach `mach` (Uj from jet_profile's construction) at the
+    shear layer, amplitude `amp` as a fraction of Uj; feedback = optional 5x5 forcing_mat too."""
+    uj = mach * np.sqrt(TJ_TINF)
+    F = np.zeros((5, 5)) if feedback is None else np.asarray(feedback, dtype=np.float64)
+    mode = 2 | (1 if feedback is not None else 0)
+    return Forcing(mode, F, z0, r0, lz, lr, amp, strouhal, float(uj), 0.0)
 
 
 # LAD constants C_mu, C_beta, C_kappa of the shock-capturing runs (the reference ships 0, 0, 0).
@@ -58,7 +87,7 @@ def uniform_z(nz: int, z_max: float = 30.0) -> np.ndarray:
     return z_max * np.arange(nz, dtype=np.float64) / (nz - 1)
 
 
-def jet_profile(r: np.ndarray, mach: float = 1.5, theta: float = 0.05, tj_tinf: float = 0.6,
+def jet_profile(r: np.ndarray, mach: float = 1.5, theta: float = 0.05, tj_tinf: float = TJ_TINF,
                 gam: float = GAMMA):
     """Top-hat jet with Crocco-Busemann density (write_soln_file.f90:34-77 construction).
 

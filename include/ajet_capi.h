@@ -55,7 +55,7 @@ typedef struct ajet_params {
   int32_t nz, nr;            /* grid points along z (i) and r (j) */
   int32_t viscous;           /* do_viscous_terms */
   int32_t shock_capturing;   /* ShockCapturing: localized artificial diffusivity */
-  int32_t forcing;           /* inflow excitation (0 = off, the reference default) */
+  int32_t forcing;           /* ControlForcing: 0 = off (the reference default), else AJET_FORCING_* bits */
   int32_t reserved;
   double gam, Re, Pr, CFL;
   double C_mu, C_beta, C_kappa;
@@ -73,6 +73,25 @@ typedef struct ajet_bc {
 typedef struct ajet_patch {
   int64_t i_start, i_end, j_start, j_end, bc, bc_dir;
 } ajet_patch;
+
+/* Forcing terms (ajet_params.forcing bits).
+ *   AJET_FORCING_FEEDBACK   ControlForcingTerms (ModControlForcing.f90:6-65):
+ *       rhs(e) += alpha * F(e,k) * G(z,r) * (q_k - q_target_k), k = 1..5, alpha = 6
+ *       (ModuleVariables.f90:184), at every point, after the viscous terms and before the
+ *       boundary conditions (ModRHS.f90:223-228);
+ *   AJET_FORCING_EXCITATION inflow excitation, an extension of this library (no counterpart in
+ *       the reference): rhs(Vz) += amp * Uj * sin(2 pi St Uj (t0 + t)) * G(z,r), with t the
+ *       time of the RK stage (time at the step start + c dt, c = 0, 1/2, 1/2, 1) and the
+ *       jet diameter as the length unit.
+ * G(z,r) = exp(-(z - z0)^2 / lz^2 - (r - r0)^2 / lr^2) in both. */
+enum { AJET_FORCING_FEEDBACK = 1, AJET_FORCING_EXCITATION = 2 };
+
+typedef struct ajet_forcing {
+  double F[5][5];            /* forcing_mat(e,k) as F[e-1][k-1] (read from Vz_Control_Vr_Feedback.dat) */
+  double z0, r0, lz, lr;     /* Gaussian centre and widths */
+  double amp, strouhal, uj;  /* excitation: amplitude (fraction of Uj), Strouhal number, jet velocity */
+  double t0;                 /* excitation: time at ajet_set_state */
+} ajet_forcing;
 
 typedef struct ajet_ctx ajet_ctx;
 
@@ -131,6 +150,11 @@ int ajet_rhs(ajet_ctx* ctx, const double* q, double* rhs);
  * ComputeArtTransCoeffs (ModShockCapturing.f90:5-106) adds for the state q; each (nz, nr). */
 int ajet_lad_coeffs(ajet_ctx* ctx, const double* q, double* mustar, double* betastar,
                     double* kappastar);
+
+/* Forcing parameters of a context created with ajet_params.forcing != 0 (the values
+ * ControlForcingTerms reads from Vz_Control_Vr_Feedback.dat, ModControlForcing.f90:13-31, and
+ * the excitation's).  Required before the first step of such a context. */
+int ajet_set_forcing(ajet_ctx* ctx, const ajet_forcing* f);
 
 /* The HIP stream the context launches on (a hipStream_t), for callers that time it. */
 void* ajet_stream(ajet_ctx* ctx);

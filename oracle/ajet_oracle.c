@@ -44,6 +44,10 @@ struct orc {
   double *pv, *pv_old, *pv_target, *rhs, *k1, *k2, *k3, *k4, *deltvec;
   double T_scalar; /* the module scalar T of ModuleVariables.f90:53 */
   double delt, time;
+  /* forcing (ajet_capi.h AJET_FORCING_*): ControlForcingTerms and the inflow excitation */
+  int forcing;
+  double fmat[5][5], fz0, fr0, flz, flr, famp, fst, fuj, ft0;
+  double t_stage; /* time of the RHS being evaluated */
   int T_set;
 };
 
@@ -445,6 +449,42 @@ static void art_trans_coeffs(orc* o) {
     }
 }
 
+/* ControlForcingTerms (ModControlForcing.f90:46-62): at every point,
+ * rhs(e) += alpha*forcing_mat(e,k)*GaussianFactor*(pv(k) - pv_target(k)), k = 1..5 in order,
+ * alpha = 6 (ModuleVariables.f90:184), GaussianFactor = exp(-(z-z0)**2/lz**2 - (r-r0)**2/lr**2).
+ * Then the inflow excitation of this project (no counterpart in the reference; ajet_capi.h):
+ * rhs(Vz) += amp*Uj*sin(2 pi St Uj (t0 + t))*GaussianFactor. */
+static void control_forcing(orc* o) {
+  const double alpha = 6.0;
+  const double w = 2.0 * 3.14159265358979323846 * o->fst * o->fuj;
+#pragma omp parallel for schedule(static)
+  for (int j = 1; j <= o->nr; ++j)
+    for (int i = 1; i <= o->nz; ++i) {
+      size_t h = HX(o, i, j);
+      double dz = o->zvec[h] - o->fz0, dr = o->rvec[h] - o->fr0;
+      double G = exp(-1.0 * ((dz * dz) / (o->flz * o->flz)) - (dr * dr) / (o->flr * o->flr));
+      if (o->forcing & AJET_FORCING_FEEDBACK)
+        for (int e = 1; e <= 5; ++e)
+          for (int k = 1; k <= 5; ++k) {
+            size_t se = SX(o, i, j, e), sk = SX(o, i, j, k);
+            o->rhs[se] = o->rhs[se] + alpha * o->fmat[e - 1][k - 1] * G * (o->pv[sk] - o->pv_target[sk]);
+          }
+      if (o->forcing & AJET_FORCING_EXCITATION) {
+        size_t s4 = SX(o, i, j, 4);
+        o->rhs[s4] = o->rhs[s4] + o->famp * o->fuj * sin(w * (o->ft0 + o->t_stage)) * G;
+      }
+    }
+}
+
+void orc_set_forcing(orc* o, int mode, const double* F, double z0, double r0, double lz, double lr, double amp,
+                     double st, double uj, double t0) {
+  o->forcing = mode;
+  for (int e = 0; e < 5; ++e)
+    for (int k = 0; k < 5; ++k) o->fmat[e][k] = F ? F[5 * e + k] : 0.0;
+  o->fz0 = z0; o->fr0 = r0; o->flz = lz; o->flr = lr;
+  o->famp = amp; o->fst = st; o->fuj = uj; o->ft0 = t0;
+}
+
 /* ---------------------------------------------------------------- boundary conditions (ModBC.f90) */
 
 typedef struct {
@@ -744,6 +784,7 @@ static void compute_rhs(orc* o) {
       }
     if (nr >= j0) o->T_scalar = o->T[HX(o, nz, nr)];
   }
+  if (o->forcing) control_forcing(o); /* ModRHS.f90:223-225 */
   boundary_conditions(o);
 }
 
@@ -879,6 +920,7 @@ int orc_rhs(orc* o, const double* q, double* rhs) {
   double Tsave = o->T_scalar;
   memcpy(o->pv, q, n * sizeof(double));
   assign_values(o);
+  o->t_stage = o->time;
   compute_rhs(o);
   memcpy(rhs, o->rhs, n * sizeof(double));
   memcpy(o->pv, save, n * sizeof(double));
@@ -926,8 +968,10 @@ int orc_rk4(orc* o) {
   memcpy(o->pv_old, o->pv, n * sizeof(double));
   double* ks[4] = {o->k1, o->k2, o->k3, o->k4};
   const double cs[3] = {0.5, 0.5, 1.0};
+  const double ct[4] = {0.0, 0.5, 0.5, 1.0};
   for (int s = 0; s < 4; ++s) {
     assign_values(o);
+    o->t_stage = o->time + ct[s] * dt;
     compute_rhs(o);
     double* k = ks[s];
     for (size_t q = 0; q < n; ++q) k[q] = dt * o->rhs[q];

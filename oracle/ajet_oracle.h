@@ -46,6 +46,9 @@ void orc_d2(orc* o, const double* field, int dirn, const double* first, double* 
 void orc_d2comp(orc* o, const double* field, int dirn, double* out);
 void orc_tgfilter(orc* o, const double* field, int dirn, int inplace, double* out);
 void orc_set_threads(int n);
+/* forcing: mode = AJET_FORCING_* bits, F = forcing_mat row-major (e, k) */
+void orc_set_forcing(orc* o, int mode, const double* F, double z0, double r0, double lz, double lr, double amp,
+                     double st, double uj, double t0);
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,7 @@ def lib():
         L.orc_stale_T.argtypes = [C.c_void_p]
         L.orc_patches.argtypes = [C.c_void_p, C.POINTER(Patch), C.c_int]
         L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_forcing.argtypes = [C.c_void_p, C.c_int, C.c_void_p] + [C.c_double] * 8
         _lib = L
     return _lib
 
@@ -103,7 +104,7 @@ class Oracle:
     """One single-domain restatement of the reference solver state."""
 
     def __init__(self, case, shock=0, c_mu=0.0, c_beta=0.0, c_kappa=0.0, viscous=1, cfl=0.5,
-                 params=None, threads=0):
+                 params=None, threads=0, forcing=None):
         L = lib()
         if threads:
             L.orc_set_threads(int(threads))
@@ -117,6 +118,11 @@ class Oracle:
         if not self.h:
             raise RuntimeError("orc_create failed")
         L.orc_set_state(self.h, F(case.q0), F(case.target))
+        if forcing is not None:
+            fm = np.ascontiguousarray(forcing.F, dtype=np.float64)
+            self._fm = fm
+            L.orc_set_forcing(self.h, int(forcing.mode), fm.ctypes.data, forcing.z0, forcing.r0, forcing.lz,
+                              forcing.lr, forcing.amp, forcing.strouhal, forcing.uj, forcing.t0)
 
     def close(self):
         if self.h:

@@ -13,6 +13,10 @@
 !   mode = 'ladrhs': ComputeRHS with the artificial coefficients C*Re*filtered*ramp injected
 !                    into mu, beta, kappa (C_mu, C_beta, C_kappa are PARAMETERs = 0 in the
 !                    reference, so the scaling is restated here) -> <out>ladrhs.bin
+!   mode = 'forcingrhs': ComputeRHS, then the reference's ControlForcingTerms (its switch
+!                    ControlForcing is a PARAMETER = 0, so the call is made here, after the
+!                    BCs instead of before them) with Vz_Control_Vr_Feedback.dat from the
+!                    working directory -> <out>forcingrhs.bin
 PROGRAM ref_harness
   USE ModuleVariables
   USE ModInputOutput
@@ -27,6 +31,7 @@ PROGRAM ref_harness
   USE ModPatch
   USE ModPlot3D_IO
   USE ModShockCapturing
+  USE ModControlForcing
   USE ModRHS
   IMPLICIT NONE
 
@@ -101,6 +106,11 @@ PROGRAM ref_harness
     CALL inject_lad()
     CALL ComputeRHS
     CALL dump_state(rhs, TRIM(h_out)//'ladrhs.bin')
+  ELSEIF (TRIM(h_mode) == 'forcingrhs') THEN
+    CALL ComputeRHS
+    iteration = iteration_start + 1
+    CALL ControlForcingTerms
+    CALL dump_state(rhs, TRIM(h_out)//'forcingrhs.bin')
   ELSEIF (TRIM(h_mode) == 'steps') THEN
     ALLOCATE(dts(h_nsteps))
     CALL ComputeTimeStep

@@ -25,7 +25,16 @@ def cases():
         "swirl64": (lambda: jc.make_jet(64, 40, seed=11, swirl=1e-3, r_max=16.0), 1, 10, ("ladrhs",)),
         "nozzle96": (lambda: jc.make_nozzle(96, 48, i_lip=24, j_in=16, j_out=22), 0, 10, ()),
         "walls64": (lambda: jc.make_jet(64, 32, seed=5, bc_lines=WALL_BC), 0, 10, ()),
+        "forcing88": (lambda: jc.make_jet(88, 32, seed=8), 0, 2, ("forcing",)),
     }
+
+
+def feedback_forcing():
+    """The forcing_mat and Gaussian of the forcing fixture (Vz_Control_Vr_Feedback.dat layout:
+    25 values, row e outer, then z0, r0, lz, lr): seeded, O(0.1), centred on the shear layer."""
+    rng = np.random.default_rng(2024)
+    F = np.round(0.1 * rng.standard_normal((5, 5)), 6)
+    return jc.Forcing(1, F, z0=3.0, r0=0.5, lz=1.5, lr=0.25)
 
 
 def input_digest(case) -> str:

@@ -27,7 +27,7 @@ from axisymmetricjet_amd import jetcase as jc  # noqa: E402
 
 HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 
-from cases import LAD_C, cases, input_digest  # noqa: E402
+from cases import LAD_C, cases, feedback_forcing, input_digest  # noqa: E402
 
 
 def run_harness(tmp, paths, case, mode, shock, nsteps=1, out="o_"):
@@ -61,6 +61,18 @@ def make(name):
             out["lad_rfiltered"] = lad[:, :, 3:6]
             out["lad_d4"] = lad[:, :, 6:12]
             out["lad_S_dil"] = lad[:, :, 12:14]
+        if "forcing" in extra:
+            f = feedback_forcing()
+            with open(os.path.join(tmp, "Vz_Control_Vr_Feedback.dat"), "w") as fh:
+                for e in range(5):
+                    for k in range(5):
+                        fh.write(f"{float(f.F[e, k])!r}\n")
+                for v in (f.z0, f.r0, f.lz, f.lr):
+                    fh.write(f"{float(v)!r}\n")
+            run_harness(tmp, paths, case, "forcingrhs", shock)
+            out["forcingrhs"] = rd("o_forcingrhs.bin").reshape((nz, nr, 5), order="F")
+            out["forcing_F"] = f.F
+            out["forcing_gauss"] = np.array([f.z0, f.r0, f.lz, f.lr])
         if "ladrhs" in extra:
             run_harness(tmp, paths, case, "ladrhs", shock)
             out["ladrhs"] = rd("o_ladrhs.bin").reshape((nz, nr, 5), order="F")
