@@ -18,7 +18,8 @@ ERRORS = {-1: "EINVAL", -2: "EGRID", -3: "EPATCH", -4: "ENODEV", -5: "EBLOWUP", 
 
 EXPORTS = ["ajet_params_default", "ajet_build_patches", "ajet_create", "ajet_destroy", "ajet_last_error",
            "ajet_set_state", "ajet_get_state", "ajet_compute_dt", "ajet_rk4_stage", "ajet_rk4_step", "ajet_run",
-           "ajet_get_dt_log", "ajet_rhs", "ajet_lad_coeffs", "ajet_set_forcing", "ajet_stream", "ajet_version"]
+           "ajet_get_dt_log", "ajet_rhs", "ajet_lad_coeffs", "ajet_set_forcing", "ajet_batch_create",
+           "ajet_batch_run", "ajet_batch_destroy", "ajet_batch_last_error", "ajet_stream", "ajet_version"]
 
 
 class Params(C.Structure):
@@ -72,6 +73,11 @@ def lib():
         L.ajet_rhs.argtypes = [vp, dp, dp]
         L.ajet_lad_coeffs.argtypes = [vp, dp, dp, dp, dp]
         L.ajet_set_forcing.argtypes = [vp, C.POINTER(ForcingC)]
+        L.ajet_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+        L.ajet_batch_run.argtypes = [vp, C.c_int32]
+        L.ajet_batch_destroy.argtypes = [vp]
+        L.ajet_batch_last_error.argtypes = [vp]
+        L.ajet_batch_last_error.restype = C.c_char_p
         L.ajet_stream.argtypes = [vp]
         L.ajet_stream.restype = vp
         L.ajet_version.restype = C.c_char_p
@@ -215,3 +221,38 @@ class Context:
 
     def stream(self):
         return lib().ajet_stream(self.h)
+
+
+class Batch:
+    """Several contexts advanced together (ajet_batch_*): one launch per kernel per RK stage over
+    all runs, later steps replayed from a captured HIP graph."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (C.c_void_p * len(self.contexts))(*[c.h.value for c in self.contexts])
+        h = C.c_void_p()
+        rc = lib().ajet_batch_create(arr, len(self.contexts), C.byref(h))
+        self.h = h
+        if rc != AJET_OK:
+            msg = lib().ajet_batch_last_error(h).decode() if h.value else ""
+            if h.value:
+                lib().ajet_batch_destroy(h)
+            self.h = None
+            raise AjetError(rc, msg)
+
+    def run(self, nsteps):
+        rc = lib().ajet_batch_run(self.h, int(nsteps))
+        if rc != AJET_OK:
+            raise AjetError(rc, lib().ajet_batch_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ajet_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

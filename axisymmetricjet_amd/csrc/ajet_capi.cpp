@@ -526,4 +526,201 @@ int ajet_lad_coeffs(ajet_ctx* c, const double* q, double* mustar, double* betast
   return AJET_OK;
 }
 
+// ---------------------------------------------------------------- ensemble
+
+struct ajet_batch {
+  std::vector<ajet_ctx*> runs;
+  int device = 0;
+  int mode = MODE_PLAIN;
+  bool force = false;
+  hipStream_t stream = nullptr, aux = nullptr;  // those of the first run
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_done = nullptr;
+  StageArgs* args[4] = {nullptr, nullptr, nullptr, nullptr};  // per stage, one entry per run
+  uint32_t *d_int = nullptr, *d_bnd = nullptr;
+  int n_int = 0, n_bnd = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};  // one step, by step parity
+  std::string err;
+};
+
+static int bfail(ajet_batch* b, int code, const std::string& msg) {
+  if (b) b->err = msg;
+  return code;
+}
+
+#define BHIPCHK(b, expr)                                                         \
+  do {                                                                           \
+    hipError_t e_ = (expr);                                                      \
+    if (e_ != hipSuccess) {                                                      \
+      if (b) (b)->err = std::string(#expr) + ": " + hipGetErrorString(e_);       \
+      return (int)e_;                                                            \
+    }                                                                            \
+  } while (0)
+
+const char* ajet_batch_last_error(const ajet_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+int ajet_batch_destroy(ajet_batch* b) {
+  if (!b) return AJET_EINVAL;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->aux) (void)hipStreamSynchronize(b->aux);
+  for (auto& g : b->graph)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (auto* a : b->args)
+    if (a) (void)hipFree(a);
+  if (b->d_int) (void)hipFree(b->d_int);
+  if (b->d_bnd) (void)hipFree(b->d_bnd);
+  for (hipEvent_t e : {b->ev_fork, b->ev_join, b->ev_done})
+    if (e) (void)hipEventDestroy(e);
+  delete b;
+  return AJET_OK;
+}
+
+int ajet_batch_create(ajet_ctx* const* runs, int32_t nruns, ajet_batch** out) {
+  if (!out) return AJET_EINVAL;
+  *out = nullptr;
+  if (!runs || nruns < 1 || nruns > 2047) return AJET_EINVAL;
+  ajet_batch* b = new ajet_batch();
+  *out = b;  // so ajet_batch_last_error can report; the caller destroys it
+  for (int r = 0; r < nruns; ++r) {
+    ajet_ctx* c = runs[r];
+    if (!c || !c->stream) return bfail(b, AJET_EINVAL, "null context");
+    if (c->device != runs[0]->device) return bfail(b, AJET_EINVAL, "runs on different devices");
+    const int m = physics_mode(c->p);
+    if (m != MODE_PLAIN && m != MODE_INVISCID)
+      return bfail(b, AJET_EUNSUPPORTED, "batched runs need shock capturing off");
+    if (m != physics_mode(runs[0]->p)) return bfail(b, AJET_EINVAL, "runs differ in do_viscous_terms");
+    if (c->p.forcing && !c->forcing_set) return bfail(b, AJET_EINVAL, "a run's forcing parameters are not set");
+    if ((int)c->s.zcuts.size() * (int)c->s.rcuts.size() >= (1 << 20)) return bfail(b, AJET_EINVAL, "grid too large");
+    b->runs.push_back(c);
+    if (c->p.forcing) b->force = true;
+  }
+  ajet_ctx* c0 = runs[0];
+  b->device = c0->device;
+  b->mode = physics_mode(c0->p);
+  b->stream = c0->stream;
+  b->aux = c0->aux;
+  BHIPCHK(b, hipSetDevice(b->device));
+  BHIPCHK(b, hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+  BHIPCHK(b, hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
+  BHIPCHK(b, hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming));
+  // per-stage arguments of every run: the buffer rotation of ajet_rk4_stage
+  for (int st = 1; st <= 4; ++st) {
+    std::vector<StageArgs> v;
+    for (ajet_ctx* c : b->runs) {
+      StageArgs a = stage_args(c);
+      a.qn = c->Q0;
+      a.acc_a = c->A;
+      a.acc_s = c->S;
+      switch (st) {
+        case 1: a.qin = c->Q0; a.qout = c->Q1; break;
+        case 2: a.qin = c->Q1; a.qout = c->Q2; break;
+        case 3: a.qin = c->Q2; a.qout = c->Q1; break;
+        default: a.qin = c->Q1; a.qout = c->Q0; break;
+      }
+      a.tiles = nullptr;
+      a.ntiles = 0;
+      v.push_back(a);
+    }
+    BHIPCHK(b, hipMalloc((void**)&b->args[st - 1], sizeof(StageArgs) * v.size()));
+    BHIPCHK(b, hipMemcpy(b->args[st - 1], v.data(), sizeof(StageArgs) * v.size(), hipMemcpyHostToDevice));
+  }
+  // work lists: interior and boundary tiles of all runs; the first boundary tile of a run writes
+  // its step scalars
+  std::vector<uint32_t> li, lb;
+  for (int r = 0; r < nruns; ++r) {
+    ajet_ctx* c = b->runs[r];
+    std::vector<int> it, bt;
+    classify_tiles(c->s, c->p.nz, c->p.nr, it, bt);
+    if (bt.empty()) return bfail(b, AJET_EINVAL, "a run has no boundary tile");
+    for (int t : it) li.push_back((uint32_t)t | ((uint32_t)r << 20));
+    for (size_t k = 0; k < bt.size(); ++k)
+      lb.push_back((uint32_t)bt[k] | ((uint32_t)r << 20) | (k == 0 ? 0x80000000u : 0u));
+  }
+  b->n_int = (int)li.size();
+  b->n_bnd = (int)lb.size();
+  BHIPCHK(b, hipMalloc((void**)&b->d_int, sizeof(uint32_t) * (li.size() + 1)));
+  BHIPCHK(b, hipMalloc((void**)&b->d_bnd, sizeof(uint32_t) * (lb.size() + 1)));
+  if (!li.empty()) BHIPCHK(b, hipMemcpy(b->d_int, li.data(), sizeof(uint32_t) * li.size(), hipMemcpyHostToDevice));
+  BHIPCHK(b, hipMemcpy(b->d_bnd, lb.data(), sizeof(uint32_t) * lb.size(), hipMemcpyHostToDevice));
+  return AJET_OK;
+}
+
+// One RK stage of every run: boundary tiles on the second stream beside the interior tiles.
+static hipError_t batch_stage(ajet_batch* b, int stage, int dt_from_dmax, int parity) {
+  hipError_t e;
+  if ((e = hipEventRecord(b->ev_fork, b->stream)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(b->aux, b->ev_fork, 0)) != hipSuccess) return e;
+  if ((e = launch_stage_batch(stage, b->mode, false, b->force, b->args[stage - 1], b->d_bnd, b->n_bnd, dt_from_dmax,
+                              parity, b->aux)) != hipSuccess)
+    return e;
+  if ((e = hipEventRecord(b->ev_join, b->aux)) != hipSuccess) return e;
+  if ((e = launch_stage_batch(stage, b->mode, true, b->force, b->args[stage - 1], b->d_int, b->n_int, dt_from_dmax,
+                              parity, b->stream)) != hipSuccess)
+    return e;
+  return hipStreamWaitEvent(b->stream, b->ev_join, 0);
+}
+
+int ajet_batch_run(ajet_batch* b, int32_t nsteps) {
+  if (!b || nsteps < 0) return AJET_EINVAL;
+  if (nsteps == 0) return AJET_OK;
+  BHIPCHK(b, hipSetDevice(b->device));
+  const int parity0 = b->runs[0]->parity;
+  for (ajet_ctx* c : b->runs) {
+    if (!c->have_state) return bfail(b, AJET_EINVAL, "a run has no state loaded");
+    if (c->parity != parity0) return bfail(b, AJET_EINVAL, "runs differ in step parity");
+  }
+  // work already queued on the runs' own streams comes first
+  for (ajet_ctx* c : b->runs)
+    if (c->stream != b->stream) BHIPCHK(b, hipStreamSynchronize(c->stream));
+  // the first step's time steps, run by run: the reduction fused into a previous last stage
+  // if there is one, else the stand-alone sweep (ajet_compute_dt)
+  for (ajet_ctx* c : b->runs) {
+    hipError_t e;
+    if (c->fused_valid) {
+      e = launch_finalize(c->sc, c->p.CFL, c->g.any_blank, c->dtlog, c->logcap, b->stream);
+    } else {
+      CflArgs a{};
+      a.q = c->Q0;
+      a.meta = c->meta;
+      a.dxidz = c->dxidz;
+      a.detadr = c->detadr;
+      a.sc = c->sc;
+      a.g = c->g;
+      a.Re = c->p.Re;
+      a.Pr = c->p.Pr;
+      e = launch_cfl(a, c->p.CFL, c->g.any_blank, c->dtlog, c->logcap, b->stream);
+    }
+    if (e != hipSuccess) return bfail(b, (int)e, std::string("time-step reduction: ") + hipGetErrorString(e));
+  }
+  int parity = parity0;
+  for (int st = 1; st <= 4; ++st) BHIPCHK(b, batch_stage(b, st, 0, parity));
+  parity ^= 1;
+  // later steps: stage 1 finishes the previous step's fused reduction; one graph per parity
+  for (int k = 1; k < nsteps; ++k) {
+    if (!b->graph[parity]) {
+      hipGraph_t g = nullptr;
+      BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e = hipSuccess;
+      for (int st = 1; st <= 4 && e == hipSuccess; ++st) e = batch_stage(b, st, 1, parity);
+      hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+      if (e != hipSuccess) return bfail(b, (int)e, std::string("capture: ") + hipGetErrorString(e));
+      BHIPCHK(b, e2);
+      e = hipGraphInstantiate(&b->graph[parity], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      BHIPCHK(b, e);
+    }
+    BHIPCHK(b, hipGraphLaunch(b->graph[parity], b->stream));
+    parity ^= 1;
+  }
+  BHIPCHK(b, hipEventRecord(b->ev_done, b->stream));
+  for (ajet_ctx* c : b->runs) {
+    if (c->stream != b->stream) BHIPCHK(b, hipStreamWaitEvent(c->stream, b->ev_done, 0));
+    c->parity = parity;
+    c->fused_valid = true;
+    c->pending_finalize = false;
+    c->dt_valid = true;
+  }
+  return AJET_OK;
+}
+
 }  // extern "C"

@@ -94,6 +94,10 @@ hipError_t launch_lad(const LadArgs& a, hipStream_t s);
 enum { MODE_PLAIN = 0, MODE_SC = 1, MODE_LAD = 2, MODE_INVISCID = 3 };
 
 hipError_t launch_stage(int stage, int mode, bool interior, const StageArgs& a, hipStream_t s);
+// Batched stages (MODE_PLAIN and MODE_INVISCID): `runs` and `list` in device memory, list
+// entries = tile | run << 20 | writer << 31.
+hipError_t launch_stage_batch(int stage, int mode, bool interior, bool force, const StageArgs* runs,
+                              const uint32_t* list, int n, int dt_from_dmax, int parity, hipStream_t s);
 hipError_t launch_cfl(const CflArgs& a, double CFL, int any_blank, double* log, long long cap, hipStream_t s);
 hipError_t launch_finalize(StepScalars* sc, double CFL, int any_blank, double* log, long long cap, hipStream_t s);
 

@@ -79,8 +79,10 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long* d
 // ShockCapturing = 1 and zero artificial terms), MODE_LAD (with mu*, beta*, kappa*),
 // MODE_INVISCID (do_viscous_terms = 0).
 // FORCE: the forcing terms are compiled in (a.forcing != 0).
+// The body works on tile t of the run described by `a`; `writer` marks the one boundary tile of
+// the run whose thread 0 updates the run's step scalars.
 template <int STAGE, int MODE, bool INT, bool FORCE>
-__global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void stage_kernel(StageArgs a) {
+__device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool writer, int dt_from_dmax, int parity) {
   constexpr bool LAD = MODE == MODE_LAD;
   constexpr bool VISC = MODE != MODE_INVISCID;
   constexpr bool SC = MODE == MODE_SC || MODE == MODE_LAD;
@@ -93,9 +95,6 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
-  const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  if (li >= a.ntiles) return;
-  const int t = a.tiles[li];
   const int tzi = t % g.ntz, tri = t / g.ntz;
   int i0, oi_lo, oi_hi, j0, oj_lo, oj_hi;
   tile_range(tzi, a.zcuts, i0, oi_lo, oi_hi);
@@ -171,28 +170,28 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
   // when asked to (every thread derives the same value; one writes it and logs it).  The
   // reduction slot is cleared in stage 2, after both stage-1 launches have read it.
   double dt = (STAGE > 0) ? a.sc->dt : 0.0;
-  if (STAGE == 1 && a.dt_from_dmax) {
+  if (STAGE == 1 && dt_from_dmax) {
     dt = dt_of_dmax(a.sc->dmax_bits, a.CFL, g.any_blank);
-    if (!INT && li == 0 && threadIdx.x == 0) {
+    if (!INT && writer && threadIdx.x == 0) {
       a.sc->dt = dt;
       const long long n = a.sc->nlog;
       if (n < a.logcap) a.dtlog[n] = dt;
       a.sc->nlog = n + 1;
     }
   }
-  if (STAGE == 2 && !INT && li == 0 && threadIdx.x == 0) a.sc->dmax_bits = 0ull;
+  if (STAGE == 2 && !INT && writer && threadIdx.x == 0) a.sc->dmax_bits = 0ull;
   double dmax = 0.0;
   double sTst = 0.0;
   // time of this stage's RHS (used by the inflow excitation): step start + c dt
-  const double tstep = a.sc->tpar[a.parity];
+  const double tstep = a.sc->tpar[parity];
   const double tst = STAGE <= 1 ? tstep : (STAGE == 4 ? tstep + dt : tstep + 0.5 * dt);
   if (STAGE == 4) {
     const double Tst = a.qin[(long long)g.iT + (long long)pitch * g.jT + 4 * fs];
     sTst = sqrt(Tst);
-    if (!INT && li == 0 && threadIdx.x == 0) {
+    if (!INT && writer && threadIdx.x == 0) {
       a.sc->T_stale = Tst;
       a.sc->time = a.sc->time + dt;
-      a.sc->tpar[a.parity ^ 1] = tstep + dt;
+      a.sc->tpar[parity ^ 1] = tstep + dt;
     }
   }
 
@@ -405,6 +404,25 @@ __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void s
   if (STAGE == 4) block_max_atomic(dmax, &a.sc->dmax_bits, red);
 }
 
+template <int STAGE, int MODE, bool INT, bool FORCE>
+__global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void stage_kernel(StageArgs a) {
+  const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (li >= a.ntiles) return;
+  stage_body<STAGE, MODE, INT, FORCE>(a, a.tiles[li], li == 0, a.dt_from_dmax, a.parity);
+}
+
+// The batched form: the work list holds (run, tile) entries of many runs, each run described by
+// its own StageArgs in `runs`; entry bits 0-19 tile, 20-30 run, 31 writer tile of the run.
+template <int STAGE, int MODE, bool INT, bool FORCE>
+__global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void stage_batch_kernel(
+    const StageArgs* __restrict__ runs, const uint32_t* __restrict__ list, int n, int dt_from_dmax, int parity) {
+  const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  if (li >= n) return;
+  const uint32_t e = list[li];
+  stage_body<STAGE, MODE, INT, FORCE>(runs[(e >> 20) & 0x7ffu], (int)(e & 0xfffffu), (e >> 31) != 0u, dt_from_dmax,
+                                      parity);
+}
+
 // ComputeTimeStep's sweep (ModInputOutput.f90:395-416) over the current state with the
 // stale scalar T kept on the device.
 __global__ __launch_bounds__(256) void cfl_kernel(CflArgs a) {
@@ -487,6 +505,38 @@ hipError_t launch_stage(int stage, int mode, bool interior, const StageArgs& a, 
       interior ? launch_mode<MODE_INVISCID, true>(stage, grid, block, a, s)
                : launch_mode<MODE_INVISCID, false>(stage, grid, block, a, s);
       break;
+  }
+  return hipGetLastError();
+}
+
+template <int MODE, bool INT, bool FORCE>
+static void launch_batch_t(int stage, dim3 grid, const StageArgs* runs, const uint32_t* list, int n, int dtm, int par,
+                           hipStream_t s) {
+  const dim3 block(STAGE_THREADS);
+  switch (stage) {
+    case 1: hipLaunchKernelGGL((stage_batch_kernel<1, MODE, INT, FORCE>), grid, block, 0, s, runs, list, n, dtm, par); break;
+    case 2: hipLaunchKernelGGL((stage_batch_kernel<2, MODE, INT, FORCE>), grid, block, 0, s, runs, list, n, dtm, par); break;
+    case 3: hipLaunchKernelGGL((stage_batch_kernel<3, MODE, INT, FORCE>), grid, block, 0, s, runs, list, n, dtm, par); break;
+    default: hipLaunchKernelGGL((stage_batch_kernel<4, MODE, INT, FORCE>), grid, block, 0, s, runs, list, n, dtm, par); break;
+  }
+}
+
+hipError_t launch_stage_batch(int stage, int mode, bool interior, bool force, const StageArgs* runs,
+                              const uint32_t* list, int n, int dt_from_dmax, int parity, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const dim3 grid(pad8(n));
+  if (mode == MODE_PLAIN) {
+    if (interior) force ? launch_batch_t<MODE_PLAIN, true, true>(stage, grid, runs, list, n, dt_from_dmax, parity, s)
+                        : launch_batch_t<MODE_PLAIN, true, false>(stage, grid, runs, list, n, dt_from_dmax, parity, s);
+    else force ? launch_batch_t<MODE_PLAIN, false, true>(stage, grid, runs, list, n, dt_from_dmax, parity, s)
+               : launch_batch_t<MODE_PLAIN, false, false>(stage, grid, runs, list, n, dt_from_dmax, parity, s);
+  } else if (mode == MODE_INVISCID) {
+    if (interior) force ? launch_batch_t<MODE_INVISCID, true, true>(stage, grid, runs, list, n, dt_from_dmax, parity, s)
+                        : launch_batch_t<MODE_INVISCID, true, false>(stage, grid, runs, list, n, dt_from_dmax, parity, s);
+    else force ? launch_batch_t<MODE_INVISCID, false, true>(stage, grid, runs, list, n, dt_from_dmax, parity, s)
+               : launch_batch_t<MODE_INVISCID, false, false>(stage, grid, runs, list, n, dt_from_dmax, parity, s);
+  } else {
+    return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -156,6 +156,21 @@ int ajet_lad_coeffs(ajet_ctx* ctx, const double* q, double* mustar, double* beta
  * the excitation's).  Required before the first step of such a context. */
 int ajet_set_forcing(ajet_ctx* ctx, const ajet_forcing* f);
 
+/* Ensemble: several contexts advanced together -- an extension with no counterpart in the
+ * reference, which runs one jet per job.  Every kernel of an RK stage is one launch over the
+ * tiles of all runs, and the steps after the first replay a captured HIP graph.  Each run keeps
+ * its own grid, state, time step, time, forcing and time-step log, and ends exactly as
+ * ajet_run on that context alone would leave it (bit for bit).  All contexts on one device,
+ * shock capturing off (the LAD kernels are not batched), the same do_viscous_terms, states
+ * loaded and, with forcing, its parameters set; all must have taken the same number of steps
+ * modulo 2.  The contexts must outlive the batch; between ajet_batch_run calls they can be used
+ * on their own. */
+typedef struct ajet_batch ajet_batch;
+int ajet_batch_create(ajet_ctx* const* runs, int32_t nruns, ajet_batch** out);
+int ajet_batch_run(ajet_batch* b, int32_t nsteps);
+int ajet_batch_destroy(ajet_batch* b);
+const char* ajet_batch_last_error(const ajet_batch* b);
+
 /* The HIP stream the context launches on (a hipStream_t), for callers that time it. */
 void* ajet_stream(ajet_ctx* ctx);
 

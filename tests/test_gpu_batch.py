@@ -1,0 +1,71 @@
+"""The ensemble (ajet_batch_*): runs of different grids, boundary layouts and forcing advanced in
+one launch per kernel per RK stage end exactly where each run alone would, bit for bit -- the
+same kernels do the same arithmetic on each run's own buffers."""
+import numpy as np
+import pytest
+
+from axisymmetricjet_amd import capi
+from axisymmetricjet_amd import jetcase as jc
+import cases as gc
+
+pytestmark = pytest.mark.gpu
+
+
+def runs():
+    return [
+        (jc.make_jet(96, 48, seed=31), jc.excitation(1.5, strouhal=0.3, amp=0.02)),
+        (jc.make_nozzle(160, 96, i_lip=40, j_in=24, j_out=30, seed=32), None),
+        (jc.make_jet(200, 64, seed=33, mach=1.3), jc.excitation(1.3, strouhal=0.7, amp=0.01,
+                                                                 feedback=gc.feedback_forcing().F)),
+        (jc.make_jet(64, 32, seed=34, bc_lines=gc.WALL_BC), None),
+    ]
+
+
+def contexts():
+    return [capi.Context(case, forcing=f) for case, f in runs()]
+
+
+def test_batch_matches_runs_alone_bit_for_bit():
+    alone = contexts()
+    for c in alone:
+        c.run(25)
+    batched = contexts()
+    b = capi.Batch(batched)
+    print("batch of", len(batched), "runs, 25 steps", flush=True)
+    b.run(25)
+    for k, (x, y) in enumerate(zip(alone, batched)):
+        dx, tx = x.dt_log()
+        dy, ty = y.dt_log()
+        assert len(dy) == 25
+        assert np.array_equal(dx, dy), k
+        assert tx == ty, k
+        assert np.array_equal(x.state(), y.state()), k
+    # the runs go on alone after the batch, and again in the batch
+    for c in alone:
+        c.run(5)
+    for c in batched:
+        c.run(3)
+    b.run(2)
+    for k, (x, y) in enumerate(zip(alone, batched)):
+        assert np.array_equal(x.state(), y.state()), k
+        assert np.array_equal(x.dt_log()[0], y.dt_log()[0]), k
+    b.close()
+
+
+def test_batch_rejects_shock_capturing_runs():
+    case = jc.make_jet(64, 32, seed=2)
+    a = capi.Context(case)
+    c = gc.LAD_C
+    l = capi.Context(case, shock_capturing=1, C_mu=c[0], C_beta=c[1], C_kappa=c[2])
+    with pytest.raises(capi.AjetError) as ei:
+        capi.Batch([a, l])
+    assert ei.value.code == -8
+
+
+def test_batch_needs_equal_step_parity():
+    case = jc.make_jet(64, 32, seed=2)
+    a, b = capi.Context(case), capi.Context(case)
+    a.run(1)
+    bt = capi.Batch([a, b])
+    with pytest.raises(capi.AjetError):
+        bt.run(1)
