@@ -68,6 +68,28 @@ def excitation(mach: float, strouhal: float, amp: float, z0: float = 1.0, r0: fl
     return Forcing(mode, F, z0, r0, lz, lr, amp, strouhal, float(uj), 0.0)
 
 
+def ensemble(n: int = 64, seed: int = 2026):
+    """The seeded ensemble of jets: grid sizes uniform over {440..2200} x {64..320} in steps of 8,
+    jet Mach uniform in [1.2, 1.8], inflow excitation with Strouhal number uniform in [0.1, 1.0]
+    and amplitude uniform in [0.1%, 5%] of Uj; the initial state is the noise-free profile, the
+    excitation being the perturbation.  Returns [(case, forcing), ...].
+
+    Under the reference's numerics (no filter with shock capturing off) these runs stay bounded
+    for a few hundred steps and then diverge: the oracle, which matches the reference bit for bit,
+    takes member 38 (520 x 216) to a non-finite state near step 650."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nz = int(rng.integers(440 // 8, 2200 // 8 + 1)) * 8
+        nr = int(rng.integers(64 // 8, 320 // 8 + 1)) * 8
+        mach = float(rng.uniform(1.2, 1.8))
+        st = float(rng.uniform(0.1, 1.0))
+        amp = float(rng.uniform(0.001, 0.05))
+        case = make_jet(nz, nr, seed=seed * 1000 + k, mach=mach, noise=0.0)
+        out.append((case, excitation(mach, strouhal=st, amp=amp)))
+    return out
+
+
 # LAD constants C_mu, C_beta, C_kappa of the shock-capturing runs (the reference ships 0, 0, 0).
 # C_beta is a tenth of the usual 1.75: the reference's time step ignores the artificial
 # viscosities, and with 1.75 the seeded grid-scale noise drives beta* to ~70 at the outflow
