@@ -26,6 +26,7 @@ struct ajet_ctx {
   double *lad = nullptr, *ladA = nullptr, *ladB = nullptr;  // mu*, beta*, kappa* and two 3-field scratch sets
   double *dzl = nullptr, *drl = nullptr, *ramp = nullptr;   // LAD grid spacings and ramp (1-D)
   uint32_t* meta = nullptr;
+  uint8_t* codes = nullptr;  // low byte of meta (the LAD filter's row codes)
   double *zc = nullptr, *rc = nullptr, *dxidz = nullptr, *ddxidxidz = nullptr, *detadr = nullptr, *ddetadetadr = nullptr;
   PatchDev* patches = nullptr;
   uint8_t* seq = nullptr;
@@ -113,7 +114,7 @@ int ajet_destroy(ajet_ctx* c) {
   void* bufs[] = {c->Q0, c->Q1, c->Q2, c->A, c->S, c->tgt, c->lad, c->ladA, c->ladB, c->dzl, c->drl, c->ramp,
                   c->meta, c->zc, c->rc, c->dxidz,
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
-                  c->int_tiles, c->bnd_tiles, c->lad_int, c->lad_bnd, c->lad_groups, c->sc, c->dtlog};
+                  c->int_tiles, c->bnd_tiles, c->lad_int, c->lad_bnd, c->lad_groups, c->codes, c->sc, c->dtlog};
   if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -191,6 +192,12 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
     for (int i = 0; i < nz; ++i) meta[(size_t)i + (size_t)g.pitch * j] = c->s.meta[(size_t)i + (size_t)nz * j];
   HIPCHK(c, hipMalloc((void**)&c->meta, sizeof(uint32_t) * meta.size()));
   HIPCHK(c, hipMemcpyAsync(c->meta, meta.data(), sizeof(uint32_t) * meta.size(), hipMemcpyHostToDevice, c->stream));
+  {
+    std::vector<uint8_t> codes(meta.size());
+    for (size_t k = 0; k < meta.size(); ++k) codes[k] = (uint8_t)(meta[k] & 255u);
+    HIPCHK(c, hipMalloc((void**)&c->codes, codes.size()));
+    HIPCHK(c, hipMemcpy(c->codes, codes.data(), codes.size(), hipMemcpyHostToDevice));
+  }
   std::vector<double> dzl, drl, ramp;
   lad_tables(c->s, nz, nr, dzl, drl, ramp);
   struct V { double** d; const std::vector<double>* h; } vs[] = {
@@ -345,6 +352,7 @@ static LadArgs lad_args(ajet_ctx* c, const double* q) {
   LadArgs l{};
   l.q = q;
   l.meta = c->meta;
+  l.codes = c->codes;
   l.dxidz = c->dxidz;
   l.detadr = c->detadr;
   l.rc = c->rc;

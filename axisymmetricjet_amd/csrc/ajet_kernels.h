@@ -65,6 +65,7 @@ struct CflArgs {
 struct LadArgs {
   const double* q;         // state whose coefficients are wanted (5 fields)
   const uint32_t* meta;
+  const uint8_t* codes;    // z row code | r row code << 4, per point (the low byte of meta)
   const double* dxidz;     // (nz)
   const double* detadr;    // (nr)
   const double* rc;        // r(j)

@@ -188,9 +188,9 @@ __global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* _
 // written back with 128-byte rows.  The walk keeps the last four filtered values and the next five inputs in
 // registers (and reads the ring 9 points ahead), so an interior step waits only on its own
 // arithmetic; closure rows read their 11
-// points from the ring (up to 10 behind or 10 ahead).  At chunk k the ring holds the columns
-// from ib - 10 + 16 (k - 1) (ib - 10 at k = 0) up to ib + 37 + 16 k, which covers every read
-// of the chunk's steps i = ib + 16 k .. ib + 16 k + 15 (columns i - 10 .. i + 10).
+// points from the ring (up to 10 behind or 10 ahead).  While walk chunk k runs, the ring holds
+// ring chunks k .. k + 2 (columns ib - 16 + 16 k .. ib + 31 + 16 k), which covers every read of
+// its steps i = ib + 16 k .. ib + 16 k + 15 (columns i - 10 .. i + 10).
 constexpr int ZWARM = 48;
 constexpr int ZLINES = LAD_LINES;
 constexpr int ZCH = 16;
@@ -222,13 +222,14 @@ __device__ __forceinline__ void zwalk_advance(ZWalk& st, double w, double xn) {
   st.X[8] = xn;
 }
 
-// 16 steps of a chunk whose points all take the interior filter.  Walk chunk k starts in ring
-// slot (10 + 16 k) mod 64, so KM = k mod 4 fixes every slot at compile time.
+// 16 steps of a chunk whose points all take the interior filter.  Walk chunk k fills ring
+// chunk k + 1, slots 16 ((k + 1) mod 4) .., so KM = (k + 1) mod 4 fixes every slot at compile
+// time.
 template <int KM>
 __device__ __forceinline__ void zwalk_interior(ZWalk& st, double (*ring)[ZLINES + 1], int lane) {
 #pragma unroll
   for (int t = 0; t < ZCH; ++t) {
-    const int sl = (10 + ZCH * KM + t) & (ZRING - 1);
+    const int sl = (ZCH * KM + t) & (ZRING - 1);
     const double xn = ring[(sl + 9) & (ZRING - 1)][lane];
     const double w = tg_interior(st);
     ring[sl][lane] = w;
@@ -257,53 +258,69 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
   double* __restrict__ out = a.out + f * fs;
   const double C = f == 0 ? a.cmu : (f == 1 ? a.cbeta : a.ckappa);
   const int lane = threadIdx.x;
-  const int c16 = lane & (ZCH - 1), l4 = lane / ZCH;  // chunk transfers: 4 lines x 16 columns per instruction
+  const int c16 = lane & (ZCH - 1), l4 = lane / ZCH;  // r filter: column c16, lines quarter l4
+  const int c8 = lane & 7, l8 = lane / 8;              // chunk transfers: 8 lines x 16 columns per instruction
   const int ib = max(0, S0 - ZWARM);                   // first point the walk filters
-  const int P0 = ib - 10;                              // column of ring slot 0: the farthest a closure row reads back
+  // column of ring slot 0: 16 before ib (closure rows read up to 10 back); ib, the segment
+  // starts and the row pitch are multiples of 16, so every chunk is one 128-byte line per line
+  const int P0 = ib - ZCH;
   // ring chunk m holds columns P0 + 16 m .. P0 + 16 m + 15; walk chunk k (points ib + 16 k ..)
-  // spans ring chunks k and k + 1.  Loads go into registers, two sets alternating: the loads of
-  // chunk k + 4 are in flight while chunk k + 3 waits to be stored.
+  // is ring chunk k + 1.  Loads go into registers, two sets alternating: the loads of chunk
+  // k + 4 are in flight while chunk k + 3 waits to be stored.  Chunks past the last column the
+  // walk reads (S1 + 9) load the last chunk it reads again instead (cache hits): loads stay
+  // unconditional, so none waits on a branch.
+  const int mlast = (S1 + 9 - P0) / ZCH;
+  // a chunk in registers: lane (c8, l8) holds columns 2 c8, 2 c8 + 1 of lines 8 r + l8
   struct Chunk {
-    double v[ZLINES / 4];
-    uint32_t cz[ZLINES / 4];
+    double2 v[ZLINES / 8];
+    uint32_t cz[ZLINES / 8];  // two codes, one per byte
   };
   Chunk ca, cb;
+  auto col = [&](int m) { return P0 + min(m, mlast) * ZCH; };  // first column ring chunk m loads
   unsigned long long interior = 0ull;  // bit (m mod 64): every z code of ring chunk m is interior
+  // Loads read clamped points (inside the row's pitch) as they are; the store replaces the
+  // points outside the grid by zero and the blank code.  No select between the two keeps a load
+  // in flight across the loop's back edge.
   auto load_chunk = [&](Chunk& c, int q) {
 #pragma unroll
-    for (int r = 0; r < ZLINES / 4; ++r) {
-      const int j = jb + 4 * r + l4, i = q + c16;
-      const bool in = j < nr && i >= 0 && i < nz;
-      const long long gi = in ? (long long)i + (long long)pitch * j : 0;
-      const double xv = x[gi];
-      const uint32_t m = a.meta[gi];
-      c.v[r] = in ? xv : 0.0;
-      c.cz[r] = in ? (m & 255u) : (ROW_BLANK | (ROW_BLANK << 4));
+    for (int r = 0; r < ZLINES / 8; ++r) {
+      const int j = min(jb + 8 * r + l8, nr - 1), i = min(max(q + 2 * c8, 0), pitch - 2);
+      const long long gi = (long long)i + (long long)pitch * j;
+      c.v[r] = *reinterpret_cast<const double2*>(x + gi);
+      c.cz[r] = *reinterpret_cast<const uint16_t*>(a.codes + gi);
     }
   };
   auto store_chunk = [&](const Chunk& c, int m) {
-    const int slot = (m * ZCH + c16) & (ZRING - 1);
+    const int slot = (m * ZCH + 2 * c8) & (ZRING - 1);
+    const int i = col(m) + 2 * c8;
+    const bool in0 = i >= 0 && i < nz, in1 = i + 1 >= 0 && i + 1 < nz;
+    constexpr uint32_t BLANK2 = ROW_BLANK | (ROW_BLANK << 4);
     bool all = true;
 #pragma unroll
-    for (int r = 0; r < ZLINES / 4; ++r) {
-      ring[slot][4 * r + l4] = c.v[r];
-      code[4 * r + l4][slot] = (uint8_t)c.cz[r];
-      all = all && (c.cz[r] & 15u) == ROW_INTERIOR;
+    for (int r = 0; r < ZLINES / 8; ++r) {
+      const int l = 8 * r + l8;
+      const bool jin = jb + l < nr;
+      const uint32_t z0 = (jin && in0) ? (c.cz[r] & 255u) : BLANK2;
+      const uint32_t z1 = (jin && in1) ? (c.cz[r] >> 8) : BLANK2;
+      ring[slot][l] = (jin && in0) ? c.v[r].x : 0.0;
+      ring[slot + 1][l] = (jin && in1) ? c.v[r].y : 0.0;
+      *reinterpret_cast<uint16_t*>(&code[l][slot]) = (uint16_t)(z0 | (z1 << 8));
+      all = all && (z0 & 15u) == ROW_INTERIOR && (z1 & 15u) == ROW_INTERIOR;
     }
     const unsigned long long bit = 1ull << (m & 63);
     interior = __all(all) ? (interior | bit) : (interior & ~bit);
   };
-  load_chunk(ca, P0);
-  load_chunk(cb, P0 + ZCH);
+  load_chunk(ca, col(0));
+  load_chunk(cb, col(1));
   store_chunk(ca, 0);
-  load_chunk(ca, P0 + 2 * ZCH);
+  load_chunk(ca, col(2));
   store_chunk(cb, 1);
-  load_chunk(cb, P0 + 3 * ZCH);  // chunk 3, stored at the end of walk chunk 0
+  load_chunk(cb, col(3));  // chunk 3, stored at the end of walk chunk 0
   store_chunk(ca, 2);
   __syncthreads();
   ZWalk st;
   {
-    const int sl = 10;  // slot of ib
+    const int sl = ZCH;  // slot of ib
     st.y1 = ring[sl - 1][lane];
     st.y2 = ring[sl - 2][lane];
     st.y3 = ring[sl - 3][lane];
@@ -336,11 +353,10 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
   // interior rows from a sliding window of 9 lines
   const int nout = out_hi - out_lo;
   const int per = (nout + 3) / 4;
-  auto rfilter_chunk = [&](int c0, int n) {
+  auto rfilter_chunk = [&](int c0, int n, double ramp) {
     const int ii = c0 + c16;
     if (c16 >= n || ii < S0 || ii >= S1) return;
     const int sl = (ii - P0) & (ZRING - 1);
-    const double ramp = a.ramp[ii];
     const int j0 = out_lo + l4 * per;
     const int j1 = min(out_hi, j0 + per);
     auto val = [&](int j) {  // z-filtered value of line j (zero outside the grid)
@@ -374,11 +390,13 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
   };
   auto walk_chunk = [&](int k, Chunk& next, const Chunk& ready) {
     const int c0 = ib + k * ZCH;  // first point of this chunk
-    load_chunk(next, P0 + (k + 4) * ZCH);
+    // the ramp before the chunk loads: waiting for it then does not wait for them
+    const double ramp = a.ramp[min(c0 + c16, nz - 1)];
+    load_chunk(next, col(k + 4));
     const int n = min(ZCH, S1 - c0);
-    const bool fast = n == ZCH && ((interior >> (k & 63)) & 1ull) && ((interior >> ((k + 1) & 63)) & 1ull);
+    const bool fast = n == ZCH && ((interior >> ((k + 1) & 63)) & 1ull);
     if (fast) {
-      switch (k & 3) {
+      switch ((k + 1) & 3) {
         case 0: zwalk_interior<0>(st, ring, lane); break;
         case 1: zwalk_interior<1>(st, ring, lane); break;
         case 2: zwalk_interior<2>(st, ring, lane); break;
@@ -388,14 +406,16 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
       for (int t = 0; t < n; ++t) general_step(c0 + t);
     }
     __syncthreads();
-    rfilter_chunk(c0, n);
+    rfilter_chunk(c0, n, ramp);
     store_chunk(ready, k + 3);  // overwrites the columns 64 behind, already filtered in r
     __syncthreads();
   };
   const int nchunks = (S1 - ib + ZCH - 1) / ZCH;
+  // chunks in pairs, the second of the last pair possibly empty (n <= 0: no step, no output),
+  // so the loop body has no branch around a walk
   for (int k = 0; k < nchunks; k += 2) {
     walk_chunk(k, ca, cb);
-    if (k + 1 < nchunks) walk_chunk(k + 1, cb, ca);
+    walk_chunk(k + 1, cb, ca);
   }
 }
 

@@ -18,6 +18,16 @@ MODULE ajet_capi
     INTEGER(C_INT64_T) :: grid, bc, bc_dir, i_start, i_end, j_start, j_end
   END TYPE ajet_bc_t
 
+  INTEGER(C_INT32_T), PARAMETER :: AJET_FORCING_FEEDBACK = 1, AJET_FORCING_EXCITATION = 2
+
+  ! ajet_forcing: F(k, e) is the C member F[e][k], the feedback weight of variable k in
+  ! equation e (the transpose of the reference's forcing_mat(e, k), ModControlForcing.f90:20-27)
+  TYPE, BIND(C) :: ajet_forcing_t
+    REAL(C_DOUBLE) :: F(5, 5)
+    REAL(C_DOUBLE) :: z0, r0, lz, lr
+    REAL(C_DOUBLE) :: amp, strouhal, uj, t0
+  END TYPE ajet_forcing_t
+
   INTERFACE
     SUBROUTINE ajet_params_default(p, nz, nr) BIND(C, name='ajet_params_default')
       IMPORT :: ajet_params_t, C_INT32_T
@@ -67,6 +77,12 @@ MODULE ajet_capi
       IMPORT :: C_INT, C_PTR
       TYPE(C_PTR), VALUE :: ctx
     END FUNCTION ajet_rk4_step
+
+    INTEGER(C_INT) FUNCTION ajet_set_forcing(ctx, f) BIND(C, name='ajet_set_forcing')
+      IMPORT :: C_INT, C_PTR, ajet_forcing_t
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ajet_forcing_t), INTENT(IN) :: f
+    END FUNCTION ajet_set_forcing
 
     INTEGER(C_INT) FUNCTION ajet_get_dt_log(ctx, dts, n, time) BIND(C, name='ajet_get_dt_log')
       IMPORT :: C_INT, C_PTR, C_INT32_T

@@ -20,6 +20,9 @@ MODULE ModuleVariables
   REAL(REAL64) :: Re = 1000.0D0, Pr = 0.72D0, CFL = 0.5D0, gam = 1.4D0
   REAL(REAL64) :: C_mu = 0.0D0, C_beta = 0.0D0, C_kappa = 0.0D0
   INTEGER :: ShockCapturing = 0, do_viscous_terms = 1
+  ! ControlForcing = 1 adds the feedback forcing of ModControlForcing.f90 read from forcing_file
+  INTEGER :: ControlForcing = 0
+  CHARACTER(LEN=256) :: forcing_file = 'Vz_Control_Vr_Feedback.dat'
   INTEGER :: noutput = 200
   INTEGER(INT64) :: no_iterations = 10000
   INTEGER(INT64) :: iteration, iteration_start
@@ -107,6 +110,36 @@ CONTAINS
     PRINT *, "DONE READING SOLUTION"
   END SUBROUTINE SolnRead
 
+  ! The feedback matrix and its Gaussian window, in the layout ControlForcingTerms reads
+  ! (ModControlForcing.f90:16-36): forcing_mat(i, j) row by row, then z0, r0, lz, lr.
+  SUBROUTINE ForcingRead(f)
+    TYPE(ajet_forcing_t), INTENT(OUT) :: f
+    REAL(REAL64) :: fm(5, 5)
+    INTEGER :: u, i, j
+    OPEN (NEWUNIT=u, FILE=TRIM(forcing_file), STATUS='old')
+    DO i = 1, 5
+      DO j = 1, 5
+        READ (u, *) fm(i, j)
+      END DO
+    END DO
+    READ (u, *) f%z0
+    READ (u, *) f%r0
+    READ (u, *) f%lz
+    READ (u, *) f%lr
+    CLOSE (u)
+    DO i = 1, 5
+      DO j = 1, 5
+        PRINT *, i, j, fm(i, j)
+      END DO
+    END DO
+    PRINT *, f%z0, f%r0, f%lz, f%lr
+    f%F = TRANSPOSE(fm)
+    f%amp = 0.0D0
+    f%strouhal = 0.0D0
+    f%uj = 0.0D0
+    f%t0 = 0.0D0
+  END SUBROUTINE ForcingRead
+
   SUBROUTINE fail_size(f)
     CHARACTER(LEN=*), INTENT(IN) :: f
     PRINT *, 'solution size differs from the grid: ', TRIM(f)
@@ -192,8 +225,10 @@ PROGRAM ModSolver
   CHARACTER(LEN=256) :: nml
   INTEGER :: u, ios
   TYPE(ajet_params_t) :: p
+  TYPE(ajet_forcing_t) :: fc
   NAMELIST /ajet_input/ GRID_FILENAME, INITIAL_CONDITION, TARGET_FILE, bc_file, out_prefix, Re, Pr, CFL, &
-                        ShockCapturing, C_mu, C_beta, C_kappa, do_viscous_terms, no_iterations, noutput
+                        ShockCapturing, C_mu, C_beta, C_kappa, do_viscous_terms, no_iterations, noutput, &
+                        ControlForcing, forcing_file
 
   ! run-time inputs the reference compiles in (ModuleVariables.f90:6-9, 69, 73, 152, 164, 177;
   ! ModSolver.f90:116), read from a namelist file (default ajet.nml) when present
@@ -220,7 +255,12 @@ PROGRAM ModSolver
   p%C_mu = C_mu
   p%C_beta = C_beta
   p%C_kappa = C_kappa
+  IF (ControlForcing == 1) p%forcing = AJET_FORCING_FEEDBACK
   CALL check(ajet_create(p, zg, rg, iblank, bclines, INT(SIZE(bclines), C_INT32_T), ctx), 'ajet_create')
+  IF (ControlForcing == 1) THEN
+    CALL ForcingRead(fc)
+    CALL check(ajet_set_forcing(ctx, fc), 'ajet_set_forcing')
+  END IF
   CALL SolnRead
   CALL check(ajet_set_state(ctx, pv, pv_target), 'ajet_set_state')
 

@@ -41,3 +41,39 @@ def test_fortran_solver_matches_reference_trajectory(tmp_path):
     err = np.abs(q - g["state"]).max()
     print(f"fortran host: state max-norm err {err:.3e}", flush=True)
     assert err <= 1e-9
+
+
+def test_fortran_solver_with_control_forcing(tmp_path):
+    """ControlForcing = 1: the host reads Vz_Control_Vr_Feedback.dat in the layout
+    ControlForcingTerms reads (ModControlForcing.f90:16-36) and the run matches the oracle's
+    forced trajectory."""
+    from oracle import pyoracle as po
+
+    case, g, shock, nsteps, extra = gc.load("forcing88")
+    f = gc.feedback_forcing()
+    nsteps = 20
+    paths = jc.write_case(str(tmp_path / "c_"), case)
+    vals = [f.F[e, k] for e in range(5) for k in range(5)] + [f.z0, f.r0, f.lz, f.lr]
+    (tmp_path / "Vz_Control_Vr_Feedback.dat").write_text("".join(f"{float(v)!r}\n" for v in vals))
+    nml = tmp_path / "ajet.nml"
+    nml.write_text(
+        "&ajet_input\n"
+        f" GRID_FILENAME='{paths['grid']}', INITIAL_CONDITION='{paths['ic']}',\n"
+        f" TARGET_FILE='{paths['target']}', bc_file='{paths['bc']}', out_prefix='{tmp_path}/',\n"
+        f" Re={case.re!r}, Pr={case.pr!r}, CFL=0.5, no_iterations={nsteps}, noutput=10,\n"
+        " ControlForcing=1\n/\n")
+    print("running ajet_solver on forcing88 with ControlForcing=1", flush=True)
+    out = subprocess.run([SOLVER, str(nml)], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    o = po.Oracle(case, forcing=f)
+    dts = o.run(nsteps)
+    lines = re.findall(r"Iteration=(\d+)\s+dt=\s*(\S+)\s+time=\s*(\S+)", out.stdout)
+    np.testing.assert_allclose([float(d) for _, d, _ in lines], dts, rtol=1e-8)
+    q, tau = jc.read_soln(str(tmp_path / f"RocFlo-CM.{nsteps:08d}.q"))
+    err = np.abs(q - o.state()).max()
+    plain = po.Oracle(case)
+    plain.run(nsteps)
+    effect = np.abs(plain.state() - o.state()).max()
+    print(f"fortran host forced: state max-norm err {err:.3e}, forcing effect {effect:.3e}", flush=True)
+    assert err <= 1e-9
+    assert effect > 1e3 * max(err, 1e-15)

@@ -57,7 +57,7 @@ def test_trajectory_vs_reference(name):
 
 
 def test_jet440_100_steps_vs_reference():
-    """configs[0]-sized run: 440 x 64, 100 RK4 steps, against the reference's own trajectory."""
+    """The reference's CPU-runnable case, 440 x 64, 100 RK4 steps, against the reference's own trajectory."""
     for name in ("jet440", "jet440_lad"):
         case, g, shock, nsteps, extra = gc.load(name)
         print(f"case {name} steps={nsteps}", flush=True)
