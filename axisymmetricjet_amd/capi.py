@@ -19,7 +19,10 @@ ERRORS = {-1: "EINVAL", -2: "EGRID", -3: "EPATCH", -4: "ENODEV", -5: "EBLOWUP", 
 EXPORTS = ["ajet_params_default", "ajet_build_patches", "ajet_create", "ajet_destroy", "ajet_last_error",
            "ajet_set_state", "ajet_get_state", "ajet_compute_dt", "ajet_rk4_stage", "ajet_rk4_step", "ajet_run",
            "ajet_get_dt_log", "ajet_rhs", "ajet_lad_coeffs", "ajet_set_forcing", "ajet_batch_create",
-           "ajet_batch_run", "ajet_batch_destroy", "ajet_batch_last_error", "ajet_stream", "ajet_version"]
+           "ajet_batch_run", "ajet_batch_destroy", "ajet_batch_last_error", "ajet_stream", "ajet_version",
+           "ajet_set_learned_lad", "ajet_lad_features", "ajet_lad_learned_coeffs", "ajet_lad_fallback_fraction",
+           "ajet_trainer_create", "ajet_trainer_load", "ajet_trainer_run", "ajet_trainer_forward",
+           "ajet_trainer_grads", "ajet_trainer_get", "ajet_trainer_destroy", "ajet_trainer_last_error"]
 
 
 class Params(C.Structure):
@@ -78,6 +81,19 @@ def lib():
         L.ajet_batch_destroy.argtypes = [vp]
         L.ajet_batch_last_error.argtypes = [vp]
         L.ajet_batch_last_error.restype = C.c_char_p
+        L.ajet_set_learned_lad.argtypes = [vp, C.POINTER(MlpC), C.c_double, C.c_double]
+        L.ajet_lad_features.argtypes = [vp, dp, vp, C.c_int32, vp]
+        L.ajet_lad_learned_coeffs.argtypes = [vp, dp, dp, dp, dp]
+        L.ajet_lad_fallback_fraction.argtypes = [vp, C.POINTER(C.c_double)]
+        L.ajet_trainer_create.argtypes = [C.c_int32, C.POINTER(MlpC), C.POINTER(vp)]
+        L.ajet_trainer_load.argtypes = [vp, vp, vp, C.c_int64]
+        L.ajet_trainer_run.argtypes = [vp, C.c_int32, C.c_float, C.c_uint64, vp]
+        L.ajet_trainer_forward.argtypes = [vp, vp, C.c_int32, vp]
+        L.ajet_trainer_grads.argtypes = [vp, vp, vp, C.c_int32, C.POINTER(MlpC), C.POINTER(C.c_double)]
+        L.ajet_trainer_get.argtypes = [vp, C.POINTER(MlpC)]
+        L.ajet_trainer_destroy.argtypes = [vp]
+        L.ajet_trainer_last_error.argtypes = [vp]
+        L.ajet_trainer_last_error.restype = C.c_char_p
         L.ajet_stream.argtypes = [vp]
         L.ajet_stream.restype = vp
         L.ajet_version.restype = C.c_char_p
@@ -88,6 +104,29 @@ def lib():
 class ForcingC(C.Structure):
     _fields_ = [("F", C.c_double * 25)] + [(n, C.c_double) for n in
                                            ("z0", "r0", "lz", "lr", "amp", "strouhal", "uj", "t0")]
+
+
+NIN, NHID, NOUT = 104, 128, 2
+MLP_FIELDS = [("w1", (NHID, NIN)), ("b1", (NHID,)), ("w2", (NHID, NHID)), ("b2", (NHID,)),
+              ("w3", (NHID, NHID)), ("b3", (NHID,)), ("w4", (NOUT, NHID)), ("b4", (NOUT,)),
+              ("shift", (NIN,)), ("scale", (NIN,)), ("out_scale", (NOUT,))]
+
+
+class MlpC(C.Structure):
+    _fields_ = [(n, C.c_float * int(np.prod(shp))) for n, shp in MLP_FIELDS]
+
+
+def mlp_to_c(model) -> MlpC:
+    """A model dict (w1 (128, 104) .. out_scale (2,), float32) as the C struct ajet_mlp."""
+    m = MlpC()
+    for n, shp in MLP_FIELDS:
+        a = np.ascontiguousarray(np.asarray(model[n], dtype=np.float32).reshape(shp)).ravel()
+        C.memmove(getattr(m, n), a.ctypes.data, a.nbytes)
+    return m
+
+
+def mlp_from_c(m: MlpC) -> dict:
+    return {n: np.frombuffer(getattr(m, n), dtype=np.float32).reshape(shp).copy() for n, shp in MLP_FIELDS}
 
 
 def default_params(nz, nr, **kw) -> Params:
@@ -219,6 +258,33 @@ class Context:
         self._check(lib().ajet_lad_coeffs(self.h, F(q), *outs), "ajet_lad_coeffs")
         return outs
 
+    def set_learned_lad(self, model, sensor_threshold: float, ramp_min: float = 0.0):
+        """Learned LAD mode (model None: the reference's LAD again)."""
+        if model is None:
+            self._check(lib().ajet_set_learned_lad(self.h, None, 0.0, 0.0), "ajet_set_learned_lad")
+            return
+        self._mlp = mlp_to_c(model)
+        self._check(lib().ajet_set_learned_lad(self.h, C.byref(self._mlp), float(sensor_threshold), float(ramp_min)),
+                    "ajet_set_learned_lad")
+
+    def lad_features(self, q, points):
+        """Raw features (n, 104) float32 of the points (n, 2) = (i, j), 0-based."""
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1, 2))
+        out = np.zeros((len(pts), NIN), dtype=np.float32)
+        self._check(lib().ajet_lad_features(self.h, F(q), pts.ctypes.data, len(pts), out.ctypes.data),
+                    "ajet_lad_features")
+        return out
+
+    def learned_coeffs(self, q):
+        outs = [np.zeros((self.nz, self.nr), order="F") for _ in range(3)]
+        self._check(lib().ajet_lad_learned_coeffs(self.h, F(q), *outs), "ajet_lad_learned_coeffs")
+        return outs
+
+    def fallback_fraction(self):
+        d = C.c_double()
+        self._check(lib().ajet_lad_fallback_fraction(self.h, C.byref(d)), "ajet_lad_fallback_fraction")
+        return d.value
+
     def stream(self):
         return lib().ajet_stream(self.h)
 
@@ -256,3 +322,69 @@ class Batch:
         except Exception:
             pass
 
+
+
+class Trainer:
+    """GPU training of the learned LAD network (ajet_trainer_*): the library's own BF16 MFMA
+    forward and backward kernels and Adam."""
+
+    def __init__(self, model, batch=4096):
+        self.batch = int(batch)
+        self._init = mlp_to_c(model)
+        h = C.c_void_p()
+        rc = lib().ajet_trainer_create(self.batch, C.byref(self._init), C.byref(h))
+        self.h = h
+        if rc != AJET_OK:
+            msg = lib().ajet_trainer_last_error(h).decode() if h.value else ""
+            if h.value:
+                lib().ajet_trainer_destroy(h)
+            self.h = None
+            raise AjetError(rc, msg)
+
+    def _check(self, rc, what):
+        if rc != AJET_OK:
+            raise AjetError(rc, f"{what}: {lib().ajet_trainer_last_error(self.h).decode()}")
+
+    def load(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        assert x.shape[1] == NIN and t.shape == (x.shape[0], NOUT)
+        self._check(lib().ajet_trainer_load(self.h, x.ctypes.data, t.ctypes.data, x.shape[0]), "ajet_trainer_load")
+
+    def run(self, steps, lr, seed=0):
+        losses = np.zeros(max(int(steps), 1))
+        self._check(lib().ajet_trainer_run(self.h, int(steps), float(lr), int(seed), losses.ctypes.data),
+                    "ajet_trainer_run")
+        return losses[:steps]
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros((x.shape[0], NOUT), dtype=np.float32)
+        self._check(lib().ajet_trainer_forward(self.h, x.ctypes.data, x.shape[0], y.ctypes.data),
+                    "ajet_trainer_forward")
+        return y
+
+    def grads(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        g = MlpC()
+        loss = C.c_double()
+        self._check(lib().ajet_trainer_grads(self.h, x.ctypes.data, t.ctypes.data, x.shape[0], C.byref(g),
+                                             C.byref(loss)), "ajet_trainer_grads")
+        return loss.value, mlp_from_c(g)
+
+    def model(self):
+        m = MlpC()
+        self._check(lib().ajet_trainer_get(self.h, C.byref(m)), "ajet_trainer_get")
+        return mlp_from_c(m)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ajet_trainer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -47,6 +47,21 @@ struct ajet_ctx {
   bool forcing_set = false;  // ajet_set_forcing was called
   int parity = 0;            // step parity (which StepScalars.tpar slot holds the step start time)
   ForcingDev fdev{};
+  // learned LAD coefficient (ajet_set_learned_lad)
+  bool learned = false;
+  __bf16* mlpW = nullptr;                  // packed BF16 weights
+  float *mlpB = nullptr, *mlpShift = nullptr, *mlpScale = nullptr;
+  float out_scale[2] = {1.0f, 1.0f};
+  double sensor_thr = 0.0, ramp_min = 0.0;
+  std::vector<int> lad_groups_h;           // host copy of the line groups
+  int nseg = 0;                            // z segments of the filter blocks
+  uint32_t* zr_flags = nullptr;            // flagged filter blocks (segment + nseg * group)
+  uint8_t *src_mask = nullptr, *str_mask = nullptr;
+  int* line_group = nullptr;               // group whose output lines hold line j
+  int infer_blocks = 0;
+  float* feat = nullptr;                   // feature hook scratch
+  int* feat_pts = nullptr;
+  int feat_cap = 0;
   std::string err;
 };
 
@@ -114,7 +129,9 @@ int ajet_destroy(ajet_ctx* c) {
   void* bufs[] = {c->Q0, c->Q1, c->Q2, c->A, c->S, c->tgt, c->lad, c->ladA, c->ladB, c->dzl, c->drl, c->ramp,
                   c->meta, c->zc, c->rc, c->dxidz,
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
-                  c->int_tiles, c->bnd_tiles, c->lad_int, c->lad_bnd, c->lad_groups, c->codes, c->sc, c->dtlog};
+                  c->int_tiles, c->bnd_tiles, c->lad_int, c->lad_bnd, c->lad_groups, c->codes, c->sc, c->dtlog,
+                  c->mlpW, c->mlpB, c->mlpShift, c->mlpScale, c->zr_flags, c->src_mask, c->str_mask,
+                  c->line_group, c->feat, c->feat_pts};
   if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -247,6 +264,7 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
     std::vector<int> gr;
     build_lad_line_groups(c->s, nz, nr, LAD_LINES, gr);
     c->n_lad_groups = (int)gr.size() / 3;
+    c->lad_groups_h = gr;
     HIPCHK(c, hipMalloc((void**)&c->lad_groups, sizeof(int) * gr.size()));
     HIPCHK(c, hipMemcpy(c->lad_groups, gr.data(), sizeof(int) * gr.size(), hipMemcpyHostToDevice));
   }
@@ -383,10 +401,48 @@ static LadArgs lad_args(ajet_ctx* c, const double* q) {
 // ComputeArtTransCoeffs after the first derivatives, ModRHS.f90:46-47).  The two stage
 // launches then write disjoint points from the same input, so the boundary tiles run on a
 // second stream beside the interior tiles; the main stream waits for both before going on.
+static MlpInfer mlp_args(ajet_ctx* c, const double* q) {
+  MlpInfer m{};
+  m.q = q;
+  m.g = c->g;
+  m.dzl = c->dzl;
+  m.drl = c->drl;
+  m.ramp = c->ramp;
+  m.rc = c->rc;
+  m.W = c->mlpW;
+  m.bias = c->mlpB;
+  m.shift = c->mlpShift;
+  m.scale = c->mlpScale;
+  m.out_scale[0] = c->out_scale[0];
+  m.out_scale[1] = c->out_scale[1];
+  m.ramp_min = c->ramp_min;
+  m.out = c->lad;
+  return m;
+}
+
+// Learned mode: the network's coefficients at the first stage of a step, then at every stage
+// the reference's LAD on the filter blocks the dilatation sensor flags.
+static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q) {
+  hipError_t e;
+  if (stage == 1 && (e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, c->stream)) != hipSuccess) return e;
+  LadArgs l = lad_args(c, q);
+  l.nseg = c->nseg;
+  if ((e = launch_lad_sensor(l, c->zr_flags, c->line_group, c->sensor_thr, c->ramp_min, c->stream)) != hipSuccess)
+    return e;
+  if ((e = launch_lad_masks(l, c->zr_flags, c->src_mask, c->str_mask, c->stream)) != hipSuccess) return e;
+  l.zr_flags = c->zr_flags;
+  l.src_mask = c->src_mask;
+  l.str_mask = c->str_mask;
+  return launch_lad(l, c->stream);
+}
+
 static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
   const int mode = physics_mode(c->p);
   hipError_t e;
-  if (mode == MODE_LAD && (e = launch_lad(lad_args(c, a.qin), c->stream)) != hipSuccess) return e;
+  if (mode == MODE_LAD) {
+    e = c->learned ? launch_learned(c, stage, a.qin) : launch_lad(lad_args(c, a.qin), c->stream);
+    if (e != hipSuccess) return e;
+  }
   if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
   a.tiles = c->bnd_tiles;
@@ -531,6 +587,103 @@ int ajet_lad_coeffs(ajet_ctx* c, const double* q, double* mustar, double* betast
                                sizeof(double) * c->g.pitch, sizeof(double) * c->g.nz, (size_t)c->g.nr,
                                hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AJET_OK;
+}
+
+// ---------------------------------------------------------------- learned LAD coefficient
+
+int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_threshold, double ramp_min) {
+  if (!c || !c->stream) return AJET_EINVAL;
+  if (!model) {
+    c->learned = false;
+    return AJET_OK;
+  }
+  if (physics_mode(c->p) != MODE_LAD)
+    return fail(c, AJET_EINVAL, "the learned coefficient needs shock capturing with non-zero C_mu, C_beta, C_kappa");
+  if (!(ramp_min >= 0.0) || sensor_threshold != sensor_threshold) return fail(c, AJET_EINVAL, "bad threshold");
+  HIPCHK(c, hipSetDevice(c->device));
+  const Geometry& g = c->g;
+  if (!c->mlpW) {
+    HIPCHK(c, hipMalloc((void**)&c->mlpW, sizeof(uint16_t) * (3 * MLP_H + 16) * MLP_K));
+    HIPCHK(c, hipMalloc((void**)&c->mlpB, sizeof(float) * (3 * MLP_H + MLP_NOUT)));
+    HIPCHK(c, hipMalloc((void**)&c->mlpShift, sizeof(float) * MLP_NIN));
+    HIPCHK(c, hipMalloc((void**)&c->mlpScale, sizeof(float) * MLP_NIN));
+    c->nseg = (g.nz + lad_zfilter_segment(g.nz) - 1) / lad_zfilter_segment(g.nz);
+    HIPCHK(c, hipMalloc((void**)&c->zr_flags, sizeof(uint32_t) * (size_t)c->nseg * c->n_lad_groups));
+    HIPCHK(c, hipMalloc((void**)&c->src_mask, (size_t)g.ntz * g.ntr));
+    HIPCHK(c, hipMalloc((void**)&c->str_mask, (size_t)g.ntz * g.ntr));
+    std::vector<int> lg((size_t)g.nr, 0);
+    for (int k = 0; k < c->n_lad_groups; ++k)
+      for (int j = c->lad_groups_h[3 * k + 1]; j < c->lad_groups_h[3 * k + 2]; ++j) lg[(size_t)j] = k;
+    HIPCHK(c, hipMalloc((void**)&c->line_group, sizeof(int) * lg.size()));
+    HIPCHK(c, hipMemcpy(c->line_group, lg.data(), sizeof(int) * lg.size(), hipMemcpyHostToDevice));
+    int cus = 0;
+    HIPCHK(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    c->infer_blocks = cus > 0 ? cus : 256;
+  }
+  std::vector<uint16_t> W;
+  std::vector<float> bias;
+  mlp_pack(*model, W, bias);
+  HIPCHK(c, hipMemcpy(c->mlpW, W.data(), sizeof(uint16_t) * W.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->mlpB, bias.data(), sizeof(float) * bias.size(), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->mlpShift, model->shift, sizeof(float) * MLP_NIN, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(c->mlpScale, model->scale, sizeof(float) * MLP_NIN, hipMemcpyHostToDevice));
+  c->out_scale[0] = model->out_scale[0];
+  c->out_scale[1] = model->out_scale[1];
+  c->sensor_thr = sensor_threshold;
+  c->ramp_min = ramp_min;
+  c->learned = true;
+  return AJET_OK;
+}
+
+int ajet_lad_features(ajet_ctx* c, const double* q, const int32_t* points, int32_t n, float* x) {
+  if (!c || !c->stream || !q || (n > 0 && (!points || !x)) || n < 0) return AJET_EINVAL;
+  for (int32_t k = 0; k < n; ++k)
+    if (points[2 * k] < 0 || points[2 * k] >= c->g.nz || points[2 * k + 1] < 0 || points[2 * k + 1] >= c->g.nr)
+      return fail(c, AJET_EINVAL, "point outside the grid");
+  if (n == 0) return AJET_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (n > c->feat_cap) {
+    if (c->feat) (void)hipFree(c->feat);
+    if (c->feat_pts) (void)hipFree(c->feat_pts);
+    c->feat = nullptr;
+    c->feat_pts = nullptr;
+    HIPCHK(c, hipMalloc((void**)&c->feat, sizeof(float) * (size_t)n * MLP_NIN));
+    HIPCHK(c, hipMalloc((void**)&c->feat_pts, sizeof(int) * 2 * (size_t)n));
+    c->feat_cap = n;
+  }
+  copy_state_in(c, c->Q2, q);
+  HIPCHK(c, hipMemcpyAsync(c->feat_pts, points, sizeof(int) * 2 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_mlp_features(mlp_args(c, c->Q2), c->feat_pts, n, c->feat, c->stream));
+  HIPCHK(c, hipMemcpyAsync(x, c->feat, sizeof(float) * (size_t)n * MLP_NIN, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AJET_OK;
+}
+
+int ajet_lad_learned_coeffs(ajet_ctx* c, const double* q, double* mustar, double* betastar, double* kappastar) {
+  if (!c || !c->stream || !q || !mustar || !betastar || !kappastar) return AJET_EINVAL;
+  if (!c->learned) return fail(c, AJET_EINVAL, "no learned model set");
+  HIPCHK(c, hipSetDevice(c->device));
+  copy_state_in(c, c->Q2, q);
+  HIPCHK(c, launch_mlp_infer(mlp_args(c, c->Q2), c->infer_blocks, c->stream));
+  double* outs[3] = {mustar, betastar, kappastar};
+  for (int f = 0; f < 3; ++f)
+    HIPCHK(c, hipMemcpy2DAsync(outs[f], sizeof(double) * c->g.nz, c->lad + (size_t)f * c->g.fstride,
+                               sizeof(double) * c->g.pitch, sizeof(double) * c->g.nz, (size_t)c->g.nr,
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AJET_OK;
+}
+
+int ajet_lad_fallback_fraction(ajet_ctx* c, double* fraction) {
+  if (!c || !fraction) return AJET_EINVAL;
+  if (!c->learned) return fail(c, AJET_EINVAL, "no learned model set");
+  std::vector<uint32_t> f((size_t)c->nseg * c->n_lad_groups);
+  HIPCHK(c, hipMemcpyAsync(f.data(), c->zr_flags, sizeof(uint32_t) * f.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  size_t k = 0;
+  for (uint32_t v : f) k += v != 0u;
+  *fraction = f.empty() ? 0.0 : (double)k / (double)f.size();
   return AJET_OK;
 }
 

@@ -94,4 +94,8 @@ void build_lad_line_groups(const Setup& s, int nz, int nr, int lines, std::vecto
 void lad_tables(const Setup& s, int nz, int nr, std::vector<double>& dz, std::vector<double>& dr,
                 std::vector<double>& ramp);
 
+// learned LAD coefficient: BF16 rounding (nearest even) and the device layout of a model
+uint16_t bf16_bits(float f);
+void mlp_pack(const ajet_mlp& m, std::vector<uint16_t>& W, std::vector<float>& bias);
+
 }  // namespace ajet

@@ -85,11 +85,95 @@ struct LadArgs {
   Geometry g;
   double gam;
   double cmu, cbeta, ckappa;  // C_mu Re, C_beta Re, C_kappa Re Pr
+  // learned mode: only the flagged z/r filter blocks (segment + nseg * group) and the tiles
+  // that feed them run (all run when these are null)
+  const uint32_t* zr_flags;
+  const uint8_t* src_mask;  // per tile: the sources are needed
+  const uint8_t* str_mask;  // per tile: the strain rate and dilatation are needed
+  int nseg;
 };
 
 hipError_t upload_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
 hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
 hipError_t launch_lad(const LadArgs& a, hipStream_t s);
+int lad_zfilter_segment(int nz);
+// the dilatation sensor of the learned mode: flags[group(j) * nseg + i / seg] |= 1 where
+// max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min (flags cleared first); then the
+// tile masks of the flagged blocks
+hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, const int* line_group, double thr, double ramp_min,
+                             hipStream_t s);
+hipError_t launch_lad_masks(const LadArgs& a, const uint32_t* flags, uint8_t* src_mask, uint8_t* str_mask,
+                            hipStream_t s);
+
+// ---- learned LAD coefficient (ajet_mlp.hip): 104 features -> 128 -> 128 -> 128 -> 2.
+constexpr int MLP_NIN = 104;       // features
+constexpr int MLP_K = 128;         // padded input width of every layer
+constexpr int MLP_H = 128;         // hidden width
+constexpr int MLP_NOUT = 2;        // beta*, kappa*
+constexpr int MLP_NOUT_PAD = 32;   // output rows as the training GEMM tiles them
+enum { MLP_EPI_FWD_GELU = 0, MLP_EPI_FWD_LAST = 1, MLP_EPI_BWD_GELU = 2, MLP_EPI_PARTIAL = 3 };
+
+// C[M x N] = A[M x K] B[N x K]^T for K-slices s = 0 .. splits-1 (A and B advance by s K).
+//   FWD_GELU: Z = C + bias (FP32), O = bf16(gelu(Z)) as M x N and OT as N x M
+//   FWD_LAST: Z = C + bias
+//   BWD_GELU: O = bf16(C gelu'(Zin)) and OT
+//   PARTIAL:  Z[s] = C (FP32 split-K partials, M x N each)
+struct MlpGemm {
+  const __bf16* A;
+  int lda;
+  const __bf16* B;
+  int ldb;
+  int M, N, K, splits;
+  const float* bias;
+  float* Z;
+  const float* Zin;
+  __bf16* O;
+  __bf16* OT;
+};
+
+// Adam over one layer's parameters p = [W (N x K), b (N)] (FP32 masters).
+struct MlpAdam {
+  float* p;
+  float* m;
+  float* v;
+  const float* partial;  // splits x N x K
+  const float* db;       // N
+  int splits, N, K;
+  __bf16* W;             // N x K
+  __bf16* WT;            // K x N
+  float* b;              // N, the bias the GEMM reads
+  float beta2, one_m_beta1, one_m_beta2, step, bc2_sqrt, eps;
+  int grad_only;
+  float* grad;           // grad_only: the reduced gradient, N x K + N
+};
+
+// Inference over the grid (and the feature hook).
+struct MlpInfer {
+  const double* q;
+  Geometry g;
+  const double* dzl;
+  const double* drl;
+  const double* ramp;
+  const double* rc;
+  const __bf16* W;        // (3 x 128 + 16) x 128: W1 (K padded), W2, W3, W4 (rows >= 2 zero)
+  const float* bias;      // b1, b2, b3 (128 each), b4 (2)
+  const float* shift;     // 104
+  const float* scale;     // 104
+  float out_scale[2];
+  double ramp_min;        // groups whose ramp stays below this get zero coefficients
+  double* out;            // 3 fields: mu* (zero), beta*, kappa*
+};
+
+hipError_t launch_mlp_gemm(int epi, const MlpGemm& g, hipStream_t s);
+hipError_t launch_mlp_loss(const float* y, const float* t, int n, __bf16* dz, __bf16* dzT, double* loss, hipStream_t s);
+hipError_t launch_mlp_bias_grad(const __bf16* dzT, int n, int ncols, float* db, hipStream_t s);
+hipError_t launch_mlp_adam(const MlpAdam& a, hipStream_t s);
+hipError_t launch_mlp_rows(const float* x, const float* t, const long long* idx, int n, __bf16* a0, __bf16* a0T,
+                           float* tb, hipStream_t s);
+hipError_t launch_mlp_sample(unsigned long long seed, long long step, long long nsamples, int n, long long* idx,
+                             hipStream_t s);
+hipError_t launch_mlp_features(const MlpInfer& a, const int* pts, int n, float* out, hipStream_t s);
+hipError_t launch_mlp_infer(const MlpInfer& a, int nblocks, hipStream_t s);
 
 // Physics variants of the stage kernel.
 enum { MODE_PLAIN = 0, MODE_SC = 1, MODE_LAD = 2, MODE_INVISCID = 3 };

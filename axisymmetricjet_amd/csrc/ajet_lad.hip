@@ -30,6 +30,7 @@ __global__ __launch_bounds__(256) void lad_strain_kernel(LadArgs a, const int* _
   const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   if (li >= ntiles) return;
   const int t = tiles[li];
+  if (a.str_mask && !a.str_mask[t]) return;
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
   for (int k = threadIdx.x; k < WZ * WR; k += blockDim.x) {
@@ -106,6 +107,7 @@ __global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* _
   const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   if (li >= ntiles) return;
   const int t = tiles[li];
+  if (a.src_mask && !a.src_mask[t]) return;
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
   for (int k = threadIdx.x; k < LWZ * LWR; k += blockDim.x) {
@@ -253,6 +255,7 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
   const int S1 = min(nz, S0 + seg);
   const int jb = a.rgroups[3 * blockIdx.y];
   const int out_lo = a.rgroups[3 * blockIdx.y + 1], out_hi = a.rgroups[3 * blockIdx.y + 2];
+  if (a.zr_flags && !a.zr_flags[blockIdx.x + (unsigned)a.nseg * blockIdx.y]) return;
   const int f = (int)blockIdx.z;
   const double* __restrict__ x = a.B + f * fs;
   double* __restrict__ out = a.out + f * fs;
@@ -417,6 +420,84 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
     walk_chunk(k, ca, cb);
     walk_chunk(k + 1, cb, ca);
   }
+}
+
+// ---- learned mode: the dilatation sensor and the tile masks of the flagged filter blocks.
+__global__ __launch_bounds__(256) void lad_clear_flags_kernel(uint32_t* flags, int n) {
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k < n) flags[k] = 0u;
+}
+
+__global__ __launch_bounds__(256) void lad_sensor_kernel(LadArgs a, uint32_t* __restrict__ flags,
+                                                         const int* __restrict__ line_group, int seg, double thr,
+                                                         double ramp_min) {
+  const Geometry& g = a.g;
+  const int nz = g.nz, nr = g.nr, pitch = g.pitch;
+  const long long fs = g.fstride;
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= (long long)nz * nr) return;
+  const int i = (int)(k % nz), j = (int)(k / nz);
+  if (!(a.ramp[i] > ramp_min)) return;
+  const int im = max(i - 1, 0), ip = min(i + 1, nz - 1), jm = max(j - 1, 0), jp = min(j + 1, nr - 1);
+  auto at = [&](int v, int ii, int jj) { return a.q[(long long)v * fs + ii + (long long)pitch * jj]; };
+  const double dVzdz = 0.5 * (at(3, ip, j) - at(3, im, j)) * a.dxidz[i];
+  const double dVrdr = 0.5 * (at(1, i, jp) - at(1, i, jm)) * a.detadr[j];
+  const double r = a.rc[j];
+  const double dil = dVzdz + dVrdr + (r > 0.0 ? at(1, i, j) / r : dVrdr);  // 2 dVr/dr on the axis
+  const double s = fmax(0.0, -dil) * fmax(a.dzl[i], a.drl[j]) / sqrt(at(4, i, j));
+  if (s > thr) {
+    uint32_t* f = &flags[line_group[j] * a.nseg + i / seg];
+    if (*f == 0u) atomicOr(f, 1u);
+  }
+}
+
+// A flagged block (segment sg, group gr) reads the sources over columns [ib - 16, S1 + 25)
+// (ib = S0 - warm-up) and lines [wb, wb + 64); a source tile reads the strain rate and
+// dilatation up to 7 points beyond itself, and a tile spans at most TZ x TR points.
+__global__ __launch_bounds__(256) void lad_mask_kernel(LadArgs a, const uint32_t* __restrict__ flags, int seg,
+                                                       uint8_t* __restrict__ src_mask, uint8_t* __restrict__ str_mask) {
+  const Geometry& g = a.g;
+  const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= g.ntz * g.ntr) return;
+  const int tzi = t % g.ntz, tri = t / g.ntz;
+  const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
+  constexpr int DZ = TZ + 8, DR = TR + 8;
+  bool src = false, str = false;
+  const int sg_lo = max(0, (i0 - DZ - 32) / seg - 1), sg_hi = min(a.nseg - 1, (i1 + DZ + 16 + ZWARM) / seg + 1);
+  for (int gr = 0; gr < a.nrgroups; ++gr) {
+    const int wb = a.rgroups[3 * gr];
+    const bool rsrc = j0 < wb + ZLINES && j1 > wb, rstr = j0 < wb + ZLINES + DR && j1 > wb - DR;
+    if (!rstr) continue;
+    for (int sg = sg_lo; sg <= sg_hi; ++sg) {
+      if (!flags[sg + a.nseg * gr]) continue;
+      const int S0 = sg * seg, S1 = min(g.nz, S0 + seg), ib = max(0, S0 - ZWARM);
+      const int zlo = ib - 16, zhi = S1 + 32;
+      src = src || (rsrc && i0 < zhi && i1 > zlo);
+      str = str || (i0 < zhi + DZ && i1 > zlo - DZ);
+    }
+  }
+  src_mask[t] = src ? 1 : 0;
+  str_mask[t] = str ? 1 : 0;
+}
+
+int lad_zfilter_segment(int nz) { return zfilter_segment(nz); }
+
+hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, const int* line_group, double thr, double ramp_min,
+                             hipStream_t s) {
+  const int nflags = a.nseg * a.nrgroups;
+  hipLaunchKernelGGL(lad_clear_flags_kernel, dim3((unsigned)((nflags + 255) / 256)), dim3(256), 0, s, flags, nflags);
+  const long long n = (long long)a.g.nz * a.g.nr;
+  hipLaunchKernelGGL(lad_sensor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, flags, line_group,
+                     zfilter_segment(a.g.nz), thr, ramp_min);
+  return hipGetLastError();
+}
+
+hipError_t launch_lad_masks(const LadArgs& a, const uint32_t* flags, uint8_t* src_mask, uint8_t* str_mask,
+                            hipStream_t s) {
+  const int nt = a.g.ntz * a.g.ntr;
+  hipLaunchKernelGGL(lad_mask_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a, flags,
+                     zfilter_segment(a.g.nz), src_mask, str_mask);
+  return hipGetLastError();
 }
 
 hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s) {

@@ -171,6 +171,66 @@ int ajet_batch_run(ajet_batch* b, int32_t nsteps);
 int ajet_batch_destroy(ajet_batch* b);
 const char* ajet_batch_last_error(const ajet_batch* b);
 
+/* ---- Learned LAD coefficient (an extension with no counterpart in the reference).
+ * A multilayer perceptron stands in for the costly part of ComputeArtTransCoeffs
+ * (ModShockCapturing.f90:5-106): from a 5 x 5 stencil of rho, Vr, Vz, T around a point (the
+ * centre as its value, the 24 neighbours as differences from it, indices clamped to the grid)
+ * and dz(i), dr(j), ramp(i), r(j) -- 104 features -- it predicts
+ *     beta* = ramp * out_scale[0] * max(y0, 0),  kappa* = ramp * out_scale[1] * max(y1, 0),
+ * mu* = 0.  Layers 104 -> 128 -> 128 -> 128 -> 2, GELU (tanh form) after the first three,
+ * weights and activations in BF16 with FP32 accumulation on the matrix cores; inputs are
+ * normalised as (x - shift) * scale in FP32 first. */
+enum { AJET_MLP_NIN = 104, AJET_MLP_HID = 128, AJET_MLP_NOUT = 2 };
+
+typedef struct ajet_mlp {
+  float w1[AJET_MLP_HID][AJET_MLP_NIN], b1[AJET_MLP_HID];
+  float w2[AJET_MLP_HID][AJET_MLP_HID], b2[AJET_MLP_HID];
+  float w3[AJET_MLP_HID][AJET_MLP_HID], b3[AJET_MLP_HID];
+  float w4[AJET_MLP_NOUT][AJET_MLP_HID], b4[AJET_MLP_NOUT];
+  float shift[AJET_MLP_NIN], scale[AJET_MLP_NIN];
+  float out_scale[AJET_MLP_NOUT];
+} ajet_mlp;
+
+/* Learned mode of a context with shock capturing on (model NULL: back to the reference's LAD).
+ * Each step, at its first RK stage, the network gives beta*, kappa* (mu* = 0) at every point
+ * where the ramp exceeds ramp_min (zero elsewhere); they are kept for the step's four stages.
+ * At every stage a dilatation sensor s = max(0, -dil) max(dz, dr) / sqrt(T) is taken, and
+ * every z/r filter block (a z segment times an r line group) holding a point with
+ * s > sensor_threshold and ramp > ramp_min gets the reference's coefficients of that stage
+ * instead: the LAD kernels run only for those blocks and the tiles that feed them.
+ * sensor_threshold < 0 reproduces the reference's LAD exactly. */
+int ajet_set_learned_lad(ajet_ctx* ctx, const ajet_mlp* model, double sensor_threshold, double ramp_min);
+
+/* Test and training hooks: the raw features (n x 104, row major) of the points (i, j), 0-based
+ * (points holds 2 n int32), of the state q (nz, nr, 5); the network's mu*, beta*, kappa* of
+ * the state q (no fallback), each (nz, nr); the fraction of filter blocks the sensor sent to
+ * the reference's LAD in the last stage taken. */
+int ajet_lad_features(ajet_ctx* ctx, const double* q, const int32_t* points, int32_t n, float* x);
+int ajet_lad_learned_coeffs(ajet_ctx* ctx, const double* q, double* mustar, double* betastar,
+                            double* kappastar);
+int ajet_lad_fallback_fraction(ajet_ctx* ctx, double* fraction);
+
+/* Training on the GPU with the library's own forward and backward kernels: minibatches of
+ * `batch` rows (a multiple of 32) drawn from the samples loaded, mean-square loss
+ * sum((y - t)^2) / (2 batch), gradients through every layer with BF16 operands and FP32
+ * accumulation, torch.optim.Adam's update (beta 0.9, 0.999, eps 1e-8) on FP32 masters. */
+typedef struct ajet_trainer ajet_trainer;
+int ajet_trainer_create(int32_t batch, const ajet_mlp* init, ajet_trainer** out);
+/* samples: x (n x 104, normalised as the model's shift and scale say) and targets t (n x 2) */
+int ajet_trainer_load(ajet_trainer* tr, const float* x, const float* t, int64_t n);
+/* steps Adam steps at learning rate lr; losses (may be NULL) receives each step's loss.  With
+ * exactly `batch` samples loaded every step takes them all (full batch, no sampling). */
+int ajet_trainer_run(ajet_trainer* tr, int32_t steps, float lr, uint64_t seed, double* losses);
+/* test hooks on n rows (n <= batch, a multiple of 32) of normalised x: the outputs y (n x 2),
+ * and the loss with its gradients (written into the w and b members of *grads) */
+int ajet_trainer_forward(ajet_trainer* tr, const float* x, int32_t n, float* y);
+int ajet_trainer_grads(ajet_trainer* tr, const float* x, const float* t, int32_t n, ajet_mlp* grads,
+                       double* loss);
+/* the current weights (shift, scale and out_scale as given at creation) */
+int ajet_trainer_get(ajet_trainer* tr, ajet_mlp* model);
+int ajet_trainer_destroy(ajet_trainer* tr);
+const char* ajet_trainer_last_error(const ajet_trainer* tr);
+
 /* The HIP stream the context launches on (a hipStream_t), for callers that time it. */
 void* ajet_stream(ajet_ctx* ctx);
 

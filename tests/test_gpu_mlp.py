@@ -1,0 +1,192 @@
+"""GPU parity of the learned LAD coefficient through the C ABI: features, the BF16 MFMA forward
+and backward kernels and Adam of the trainer, the fused inference kernel, and the learned mode
+of the time step against the oracle (oracle/mlp_oracle.py) and the reference's LAD."""
+import numpy as np
+import pytest
+
+from axisymmetricjet_amd import capi
+from axisymmetricjet_amd import jetcase as jc
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+LAD = dict(shock_capturing=1, C_mu=jc.LAD_C[0], C_beta=jc.LAD_C[1], C_kappa=jc.LAD_C[2])
+
+
+def rand_model(seed, shift_scale=False):
+    m = mo.init_model(seed)
+    rng = np.random.default_rng(seed + 50)
+    for k in ("b1", "b2", "b3", "b4"):
+        m[k] = (0.1 * rng.standard_normal(m[k].shape)).astype(np.float32)
+    m["w4"] = (0.2 * rng.standard_normal(m["w4"].shape)).astype(np.float32)
+    if shift_scale:
+        m["shift"] = rng.standard_normal(mo.NIN).astype(np.float32) * 0.01
+        m["scale"] = rng.uniform(0.5, 50.0, mo.NIN).astype(np.float32)
+        m["out_scale"] = np.array([3e-3, 7e-4], np.float32)
+    return m
+
+
+def close_bf16(got, want, what, tol_max=2e-2, tol_mean=2e-3):
+    """BF16 activations: a different FP32 accumulation order moves an occasional rounding by
+    one BF16 ulp (2^-8 relative), so the bar is normwise, with a tighter mean."""
+    scale = np.abs(want).max()
+    err = np.abs(got - want)
+    print(f"{what}: max {err.max() / scale:.2e}, mean {err.mean() / max(np.abs(want).mean(), 1e-30):.2e}", flush=True)
+    assert err.max() <= tol_max * scale
+    assert err.mean() <= tol_mean * np.abs(want).mean()
+
+
+def test_features_match_oracle():
+    case = jc.make_jet(96, 40, seed=21)
+    q = np.array(case.q0, order="F")
+    q[:, :, 0] += 1e-2 * np.sin(np.arange(96))[:, None]
+    ctx = capi.Context(case, **LAD)
+    rng = np.random.default_rng(1)
+    pts = np.concatenate([np.stack([rng.integers(0, 96, 500), rng.integers(0, 40, 500)], 1),
+                          [[0, 0], [95, 39], [1, 38], [94, 1], [50, 0]]]).astype(np.int32)
+    got = ctx.lad_features(q, pts)
+    want = mo.features(q, case.z[:, 0], case.r[0, :], pts)
+    assert np.array_equal(got[:, :102], want[:, :102])
+    assert np.array_equal(got[:, 103], want[:, 103])
+    np.testing.assert_allclose(got[:, 102], want[:, 102], rtol=1e-6, atol=1e-30)  # tanh of two libms
+
+
+def test_trainer_forward_matches_oracle():
+    m = rand_model(1)
+    x = np.random.default_rng(2).standard_normal((256, mo.NIN)).astype(np.float32)
+    tr = capi.Trainer(m, batch=256)
+    y = tr.forward(x)
+    want, _ = mo.forward(m, x)
+    close_bf16(y, want, "forward")
+
+
+def test_trainer_gradients_match_oracle():
+    m = rand_model(3)
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((512, mo.NIN)).astype(np.float32)
+    t = rng.standard_normal((512, mo.NOUT)).astype(np.float32)
+    tr = capi.Trainer(m, batch=512)
+    loss, g = tr.grads(x, t)
+    lref, gref = mo.loss_and_grads(m, x, t)
+    print(f"loss {loss:.6e} oracle {lref:.6e}", flush=True)
+    assert abs(loss - lref) <= 2e-3 * lref
+    for k in mo.PARAMS:
+        close_bf16(g[k], gref[k], f"grad {k}", tol_max=5e-2, tol_mean=1e-2)
+
+
+def test_adam_step_matches_oracle():
+    """Full batch: the GPU step from its own (deterministic) gradients against the oracle's
+    Adam applied to the same gradients."""
+    m = rand_model(5)
+    rng = np.random.default_rng(6)
+    x = rng.standard_normal((256, mo.NIN)).astype(np.float32)
+    t = rng.standard_normal((256, mo.NOUT)).astype(np.float32)
+    tr = capi.Trainer(m, batch=256)
+    tr.load(x, t)
+    ref = {k: np.array(v) for k, v in m.items()}
+    state = {}
+    for step in range(3):
+        _, g = tr.grads(x, t)
+        losses = tr.run(1, lr=1e-3)
+        mo.adam_step(ref, g, state, lr=1e-3)
+        got = tr.model()
+        for k in mo.PARAMS:
+            np.testing.assert_allclose(got[k], ref[k], rtol=1e-6, atol=1e-9, err_msg=f"step {step} {k}")
+        assert losses.shape == (1,) and np.isfinite(losses[0])
+
+
+def test_training_reduces_the_loss():
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((16384, mo.NIN)).astype(np.float32)
+    w = rng.standard_normal((mo.NIN, 2)) / np.sqrt(mo.NIN)
+    t = np.stack([np.tanh(x @ w[:, 0]), 0.5 * (x @ w[:, 1]) ** 2], 1).astype(np.float32)
+    tr = capi.Trainer(mo.init_model(8), batch=1024)
+    tr.load(x, t)
+    losses = tr.run(400, lr=2e-3, seed=3)
+    print(f"loss {losses[:10].mean():.4e} -> {losses[-10:].mean():.4e}", flush=True)
+    assert losses[-20:].mean() < 0.2 * losses[:10].mean()
+
+
+def test_inference_kernel_matches_oracle():
+    case = jc.make_jet(200, 48, seed=22)
+    q = np.array(case.q0, order="F")
+    rng = np.random.default_rng(9)
+    q[:, :, 1] += 1e-2 * rng.standard_normal(q.shape[:2])
+    q[:, :, 3] += 1e-2 * rng.standard_normal(q.shape[:2])
+    m = rand_model(10, shift_scale=True)
+    ctx = capi.Context(case, **LAD)
+    ramp_min = 1e-6
+    ctx.set_learned_lad(m, sensor_threshold=1e30, ramp_min=ramp_min)
+    mu, beta, kappa = ctx.learned_coeffs(q)
+    nz, nr = case.nz, case.nr
+    ii, jj = np.meshgrid(np.arange(nz), np.arange(nr), indexing="ij")
+    pts = np.stack([ii.ravel(order="F"), jj.ravel(order="F")], 1)
+    x = mo.features(q, case.z[:, 0], case.r[0, :], pts)
+    _, _, ramp = mo.lad_spacings(case.z[:, 0], case.r[0, :])
+    b_ref, k_ref = mo.coefficients(m, x, ramp[pts[:, 0]])
+    # groups of 16 points along z whose ramp stays below ramp_min are zero
+    last = np.minimum((np.arange(nz) // 16) * 16 + 15, nz - 1)
+    live = (ramp[last] >= ramp_min)[:, None] & np.ones((1, nr), bool)
+    b_ref = np.where(live, b_ref.reshape((nz, nr), order="F"), 0.0)
+    k_ref = np.where(live, k_ref.reshape((nz, nr), order="F"), 0.0)
+    assert np.all(mu == 0.0)
+    assert np.all(beta[~live] == 0.0) and np.all(kappa[~live] == 0.0)
+    assert np.abs(b_ref).max() > 0 and np.abs(k_ref).max() > 0
+    close_bf16(beta, b_ref, "beta*")
+    close_bf16(kappa, k_ref, "kappa*")
+
+
+def test_negative_threshold_is_the_reference_lad_bit_for_bit():
+    """Every flagged block takes the reference's coefficients at every stage: the learned mode
+    with a negative sensor threshold is the plain LAD run, bit for bit."""
+    case = jc.make_jet(440, 64, seed=1234)
+    a = capi.Context(case, **LAD)
+    b = capi.Context(case, **LAD)
+    b.set_learned_lad(rand_model(11, shift_scale=True), sensor_threshold=-1.0, ramp_min=0.0)
+    a.run(12)
+    b.run(12)
+    assert b.fallback_fraction() > 0.5
+    assert np.array_equal(a.dt_log()[0], b.dt_log()[0])
+    assert np.array_equal(a.state(), b.state())
+
+
+def test_learned_mode_without_fallback_steps_and_differs():
+    case = jc.make_jet(440, 64, seed=1234)
+    a = capi.Context(case, **LAD)
+    b = capi.Context(case, **LAD)
+    m = rand_model(12, shift_scale=True)
+    m["out_scale"] = np.array([1e-6, 1e-7], np.float32)  # a tiny coefficient: stable, but not the LAD's
+    b.set_learned_lad(m, sensor_threshold=1e30, ramp_min=1e-8)
+    a.run(10)
+    b.run(10)
+    assert b.fallback_fraction() == 0.0
+    sa, sb = a.state(), b.state()
+    assert np.all(np.isfinite(sb))
+    assert 0 < np.abs(sa - sb).max() < 1e-2
+    b.set_learned_lad(None, 0.0)
+    b.set_state(case.q0, case.target)
+    a.set_state(case.q0, case.target)
+    a.run(3)
+    b.run(3)
+    assert np.array_equal(a.state(), b.state())
+
+
+def test_sensor_flags_a_compression():
+    """A compression (Vz falling along z) where the ramp is on sends its blocks to the
+    reference's LAD; the same state without it flags none at a high threshold."""
+    case = jc.make_jet(440, 64, seed=5)
+    z = case.z[:, 0]
+    q = np.array(case.q0, order="F")
+    bump = -0.3 * np.tanh((z - 20.0) / 0.2)
+    q[:, :, 3] += bump[:, None]
+    ctx = capi.Context(case, **LAD)
+    ctx.set_state(q, case.target)
+    ctx.set_learned_lad(rand_model(13, shift_scale=True), sensor_threshold=0.05, ramp_min=1e-8)
+    ctx.run(1)
+    frac = ctx.fallback_fraction()
+    print(f"fallback fraction with the compression: {frac:.3f}", flush=True)
+    assert 0.0 < frac < 0.5
+    ctx2 = capi.Context(case, **LAD)
+    ctx2.set_learned_lad(rand_model(13, shift_scale=True), sensor_threshold=0.05, ramp_min=1e-8)
+    ctx2.run(1)
+    assert ctx2.fallback_fraction() < frac
