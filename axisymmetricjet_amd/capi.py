@@ -22,7 +22,8 @@ EXPORTS = ["ajet_params_default", "ajet_build_patches", "ajet_create", "ajet_des
            "ajet_batch_run", "ajet_batch_destroy", "ajet_batch_last_error", "ajet_stream", "ajet_version",
            "ajet_set_learned_lad", "ajet_lad_features", "ajet_lad_learned_coeffs", "ajet_lad_fallback_fraction",
            "ajet_trainer_create", "ajet_trainer_load", "ajet_trainer_run", "ajet_trainer_forward",
-           "ajet_trainer_grads", "ajet_trainer_get", "ajet_trainer_destroy", "ajet_trainer_last_error"]
+           "ajet_trainer_grads", "ajet_trainer_get", "ajet_trainer_destroy", "ajet_trainer_last_error",
+           "ajet_set_probes", "ajet_probe"]
 
 
 class Params(C.Structure):
@@ -81,7 +82,7 @@ def lib():
         L.ajet_batch_destroy.argtypes = [vp]
         L.ajet_batch_last_error.argtypes = [vp]
         L.ajet_batch_last_error.restype = C.c_char_p
-        L.ajet_set_learned_lad.argtypes = [vp, C.POINTER(MlpC), C.c_double, C.c_double]
+        L.ajet_set_learned_lad.argtypes = [vp, C.POINTER(MlpC), C.c_double, C.c_double, C.c_int32]
         L.ajet_lad_features.argtypes = [vp, dp, vp, C.c_int32, vp]
         L.ajet_lad_learned_coeffs.argtypes = [vp, dp, dp, dp, dp]
         L.ajet_lad_fallback_fraction.argtypes = [vp, C.POINTER(C.c_double)]
@@ -94,6 +95,8 @@ def lib():
         L.ajet_trainer_destroy.argtypes = [vp]
         L.ajet_trainer_last_error.argtypes = [vp]
         L.ajet_trainer_last_error.restype = C.c_char_p
+        L.ajet_set_probes.argtypes = [vp, vp, C.c_int32]
+        L.ajet_probe.argtypes = [vp, vp]
         L.ajet_stream.argtypes = [vp]
         L.ajet_stream.restype = vp
         L.ajet_version.restype = C.c_char_p
@@ -258,14 +261,14 @@ class Context:
         self._check(lib().ajet_lad_coeffs(self.h, F(q), *outs), "ajet_lad_coeffs")
         return outs
 
-    def set_learned_lad(self, model, sensor_threshold: float, ramp_min: float = 0.0):
+    def set_learned_lad(self, model, sensor_threshold: float, ramp_min: float = 0.0, refresh: int = 1):
         """Learned LAD mode (model None: the reference's LAD again)."""
         if model is None:
-            self._check(lib().ajet_set_learned_lad(self.h, None, 0.0, 0.0), "ajet_set_learned_lad")
+            self._check(lib().ajet_set_learned_lad(self.h, None, 0.0, 0.0, 1), "ajet_set_learned_lad")
             return
         self._mlp = mlp_to_c(model)
-        self._check(lib().ajet_set_learned_lad(self.h, C.byref(self._mlp), float(sensor_threshold), float(ramp_min)),
-                    "ajet_set_learned_lad")
+        self._check(lib().ajet_set_learned_lad(self.h, C.byref(self._mlp), float(sensor_threshold), float(ramp_min),
+                                               int(refresh)), "ajet_set_learned_lad")
 
     def lad_features(self, q, points):
         """Raw features (n, 104) float32 of the points (n, 2) = (i, j), 0-based."""
@@ -284,6 +287,17 @@ class Context:
         d = C.c_double()
         self._check(lib().ajet_lad_fallback_fraction(self.h, C.byref(d)), "ajet_lad_fallback_fraction")
         return d.value
+
+    def set_probes(self, points):
+        self._probe_pts = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1, 2))
+        self._check(lib().ajet_set_probes(self.h, self._probe_pts.ctypes.data, len(self._probe_pts)),
+                    "ajet_set_probes")
+
+    def probe(self):
+        """(n, 5) primitives at the probe points of the current state."""
+        out = np.zeros((len(self._probe_pts), 5))
+        self._check(lib().ajet_probe(self.h, out.ctypes.data), "ajet_probe")
+        return out
 
     def stream(self):
         return lib().ajet_stream(self.h)

@@ -53,6 +53,8 @@ struct ajet_ctx {
   float *mlpB = nullptr, *mlpShift = nullptr, *mlpScale = nullptr;
   float out_scale[2] = {1.0f, 1.0f};
   double sensor_thr = 0.0, ramp_min = 0.0;
+  int refresh = 1;                         // network every refresh steps
+  long long learned_steps = 0;             // steps since the network last ran (0: run it next)
   std::vector<int> lad_groups_h;           // host copy of the line groups
   int nseg = 0;                            // z segments of the filter blocks
   uint32_t* zr_flags = nullptr;            // flagged filter blocks (segment + nseg * group)
@@ -62,6 +64,9 @@ struct ajet_ctx {
   float* feat = nullptr;                   // feature hook scratch
   int* feat_pts = nullptr;
   int feat_cap = 0;
+  int* probe_pts = nullptr;                // ajet_probe: points and values on the device
+  double* probe_out = nullptr;
+  int probe_n = 0;
   std::string err;
 };
 
@@ -131,7 +136,7 @@ int ajet_destroy(ajet_ctx* c) {
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
                   c->int_tiles, c->bnd_tiles, c->lad_int, c->lad_bnd, c->lad_groups, c->codes, c->sc, c->dtlog,
                   c->mlpW, c->mlpB, c->mlpShift, c->mlpScale, c->zr_flags, c->src_mask, c->str_mask,
-                  c->line_group, c->feat, c->feat_pts};
+                  c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out};
   if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -290,6 +295,7 @@ int ajet_set_state(ajet_ctx* c, const double* pv, const double* pv_target) {
   HIPCHK(c, hipSetDevice(c->device));
   copy_state_in(c, c->Q0, pv);
   if (pv_target) copy_state_in(c, c->tgt, pv_target);
+  c->learned_steps = 0;
   StepScalars z{};
   HIPCHK(c, hipMemcpyAsync(c->sc, &z, sizeof z, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -424,7 +430,10 @@ static MlpInfer mlp_args(ajet_ctx* c, const double* q) {
 // the reference's LAD on the filter blocks the dilatation sensor flags.
 static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q) {
   hipError_t e;
-  if (stage == 1 && (e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, c->stream)) != hipSuccess) return e;
+  if (stage == 1 && c->learned_steps % c->refresh == 0 &&
+      (e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, c->stream)) != hipSuccess)
+    return e;
+  if (stage == 4) ++c->learned_steps;
   LadArgs l = lad_args(c, q);
   l.nseg = c->nseg;
   if ((e = launch_lad_sensor(l, c->zr_flags, c->line_group, c->sensor_thr, c->ramp_min, c->stream)) != hipSuccess)
@@ -580,6 +589,7 @@ int ajet_lad_coeffs(ajet_ctx* c, const double* q, double* mustar, double* betast
   if (c->p.shock_capturing != 1) return fail(c, AJET_EINVAL, "shock capturing is off in this context");
   HIPCHK(c, hipSetDevice(c->device));
   copy_state_in(c, c->Q2, q);
+  c->learned_steps = 0;  // the coefficient buffer is overwritten: the next step recomputes
   HIPCHK(c, launch_lad(lad_args(c, c->Q2), c->stream));
   double* outs[3] = {mustar, betastar, kappastar};
   for (int f = 0; f < 3; ++f)
@@ -590,10 +600,41 @@ int ajet_lad_coeffs(ajet_ctx* c, const double* q, double* mustar, double* betast
   return AJET_OK;
 }
 
+int ajet_set_probes(ajet_ctx* c, const int32_t* points, int32_t n) {
+  if (!c || !c->stream || n < 0 || (n > 0 && !points)) return AJET_EINVAL;
+  for (int32_t k = 0; k < n; ++k)
+    if (points[2 * k] < 0 || points[2 * k] >= c->g.nz || points[2 * k + 1] < 0 || points[2 * k + 1] >= c->g.nr)
+      return fail(c, AJET_EINVAL, "probe outside the grid");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->probe_pts) (void)hipFree(c->probe_pts);
+  if (c->probe_out) (void)hipFree(c->probe_out);
+  c->probe_pts = nullptr;
+  c->probe_out = nullptr;
+  c->probe_n = n;
+  if (n == 0) return AJET_OK;
+  HIPCHK(c, hipMalloc((void**)&c->probe_pts, sizeof(int) * 2 * (size_t)n));
+  HIPCHK(c, hipMalloc((void**)&c->probe_out, sizeof(double) * 5 * (size_t)n));
+  HIPCHK(c, hipMemcpy(c->probe_pts, points, sizeof(int) * 2 * (size_t)n, hipMemcpyHostToDevice));
+  return AJET_OK;
+}
+
+int ajet_probe(ajet_ctx* c, double* values) {
+  if (!c || !c->stream || !values) return AJET_EINVAL;
+  if (c->probe_n == 0) return AJET_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, launch_probe(c->Q0, c->g, c->probe_pts, c->probe_n, c->probe_out, c->stream));
+  HIPCHK(c, hipMemcpyAsync(values, c->probe_out, sizeof(double) * 5 * (size_t)c->probe_n, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return AJET_OK;
+}
+
 // ---------------------------------------------------------------- learned LAD coefficient
 
-int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_threshold, double ramp_min) {
+int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_threshold, double ramp_min,
+                         int32_t refresh) {
   if (!c || !c->stream) return AJET_EINVAL;
+  c->learned_steps = 0;
   if (!model) {
     c->learned = false;
     return AJET_OK;
@@ -601,6 +642,7 @@ int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_thres
   if (physics_mode(c->p) != MODE_LAD)
     return fail(c, AJET_EINVAL, "the learned coefficient needs shock capturing with non-zero C_mu, C_beta, C_kappa");
   if (!(ramp_min >= 0.0) || sensor_threshold != sensor_threshold) return fail(c, AJET_EINVAL, "bad threshold");
+  if (refresh < 1) return fail(c, AJET_EINVAL, "refresh must be at least 1");
   HIPCHK(c, hipSetDevice(c->device));
   const Geometry& g = c->g;
   if (!c->mlpW) {
@@ -632,6 +674,7 @@ int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_thres
   c->out_scale[1] = model->out_scale[1];
   c->sensor_thr = sensor_threshold;
   c->ramp_min = ramp_min;
+  c->refresh = refresh;
   c->learned = true;
   return AJET_OK;
 }
@@ -665,6 +708,7 @@ int ajet_lad_learned_coeffs(ajet_ctx* c, const double* q, double* mustar, double
   if (!c->learned) return fail(c, AJET_EINVAL, "no learned model set");
   HIPCHK(c, hipSetDevice(c->device));
   copy_state_in(c, c->Q2, q);
+  c->learned_steps = 0;
   HIPCHK(c, launch_mlp_infer(mlp_args(c, c->Q2), c->infer_blocks, c->stream));
   double* outs[3] = {mustar, betastar, kappastar};
   for (int f = 0; f < 3; ++f)

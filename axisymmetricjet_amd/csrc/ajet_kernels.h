@@ -185,5 +185,6 @@ hipError_t launch_stage_batch(int stage, int mode, bool interior, bool force, co
                               const uint32_t* list, int n, int dt_from_dmax, int parity, hipStream_t s);
 hipError_t launch_cfl(const CflArgs& a, double CFL, int any_blank, double* log, long long cap, hipStream_t s);
 hipError_t launch_finalize(StepScalars* sc, double CFL, int any_blank, double* log, long long cap, hipStream_t s);
+hipError_t launch_probe(const double* q, const Geometry& g, const int* pts, int n, double* out, hipStream_t s);
 
 }  // namespace ajet

@@ -556,4 +556,19 @@ hipError_t launch_finalize(StepScalars* sc, double CFL, int any_blank, double* l
   return hipGetLastError();
 }
 
+// The five primitives at n points (i, j), 0-based: out[5 k + v].
+__global__ __launch_bounds__(256) void probe_kernel(const double* __restrict__ q, Geometry g, const int* __restrict__ pts,
+                                                    int n, double* __restrict__ out) {
+  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (k >= 5 * n) return;
+  const int p = k / 5, v = k - 5 * p;
+  out[k] = q[(long long)v * g.fstride + pts[2 * p] + (long long)g.pitch * pts[2 * p + 1]];
+}
+
+hipError_t launch_probe(const double* q, const Geometry& g, const int* pts, int n, double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(probe_kernel, dim3((unsigned)((5 * n + 255) / 256)), dim3(256), 0, s, q, g, pts, n, out);
+  return hipGetLastError();
+}
+
 }  // namespace ajet

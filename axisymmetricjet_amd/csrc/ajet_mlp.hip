@@ -29,10 +29,13 @@ constexpr float A_GELU = 0.044715f;
 constexpr float LOG2E = 1.4426950408889634f;
 
 // GELU in its tanh form, evaluated as x s with s = 1 / (1 + exp(-2u)) = (1 + tanh u) / 2,
-// u = sqrt(2/pi) (x + 0.044715 x^3); the hardware exp2 and reciprocal.
+// u = sqrt(2/pi) (x + 0.044715 x^3), so -2u log2(e) = x (G1 + G2 x^2); the hardware exp2 and
+// reciprocal.
+constexpr float G1 = -2.0f * LOG2E * K_GELU;
+constexpr float G2 = -2.0f * LOG2E * K_GELU * A_GELU;
 __device__ __forceinline__ float gelu_sig(float x) {
-  const float u = K_GELU * (x + A_GELU * x * x * x);
-  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-2.0f * LOG2E * u));
+  const float w = x * __builtin_fmaf(G2, x * x, G1);
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(w));
 }
 __device__ __forceinline__ float gelu_f(float x) { return x * gelu_sig(x); }
 // d/dx [x s(x)] = s + 2 x s (1 - s) sqrt(2/pi) (1 + 3 0.044715 x^2)
@@ -225,22 +228,33 @@ __global__ __launch_bounds__(256) void mlp_features_kernel(MlpInfer a, const int
   out[k] = mlp_raw_feature(a, pts[2 * p], pts[2 * p + 1], f);
 }
 
-// ---- inference.  A wave takes 16 points (i0 .. i0+15, j) at a time: lane (p, q) = (l mod 16,
-// l / 16) builds features q, q+4, .. of point p into the wave's activation rows, then each layer
-// is 8 output tiles of 16 x 16 over K = 128 in four 32-deep MFMA steps; the epilogue adds the
-// bias, applies GELU and writes the BF16 activations back over the layer's input.
-constexpr int WROW = MLP_K + 8;  // padded LDS row (BF16 elements)
+// ---- inference.  A wave takes 16 points (i0 .. i0+15, j) at a time.  Lane (p, q) = (l mod 16,
+// l / 16) loads the 5 x 5 stencil of variable q around point p and writes features
+// 25 q .. 25 q + 24 of that point (and metric 100 + q) into the wave's activation rows.  Each
+// layer is 8 tiles of 16 outputs x 16 points: A = 16 weight rows, B = the activation rows of the
+// 16 points, K = 128 in four 32-deep MFMA steps; the accumulator of lane (p, q) then holds
+// outputs 16 t + 4 q .. + 3 of point p, which the epilogue (bias, GELU, BF16) writes back over
+// the layer's input as one 8-byte store.
+constexpr int WROW = MLP_K + 8;  // padded LDS row (BF16 elements): 16-byte reads conflict-free
 constexpr int INFER_WAVES = 8;
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void wave_order() {
+  // the activation rows pass between lanes of one wave only: LDS keeps the wave's order, the
+  // barrier keeps the compiler's
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __global__ __launch_bounds__(64 * INFER_WAVES) void mlp_infer_kernel(MlpInfer a) {
   __shared__ __bf16 sW[3 * MLP_H + 16][WROW];            // W1 (K padded), W2, W3, W4 (rows >= 2 zero)
   __shared__ __bf16 sAct[INFER_WAVES][16][WROW];
-  __shared__ float sBias[3 * MLP_H + MLP_NOUT];
+  __shared__ __attribute__((aligned(16))) float sBias[3 * MLP_H + MLP_NOUT];
   __shared__ float sShift[MLP_NIN], sScale[MLP_NIN];
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
-  // weights: rows of 128 BF16 as 16-byte pieces
   for (int k = (int)threadIdx.x; k < (3 * MLP_H + 16) * (MLP_K / 8); k += blockDim.x) {
     const int r = k / (MLP_K / 8), c = (k % (MLP_K / 8)) * 8;
     *reinterpret_cast<bf16x8*>(&sW[r][c]) = *reinterpret_cast<const bf16x8*>(a.W + (long long)r * MLP_K + c);
@@ -254,6 +268,7 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void mlp_infer_kernel(MlpInfer a)
   const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
   const int p = lane & 15, q = lane >> 4;
   __bf16(*act)[WROW] = sAct[wave];
+  const double* __restrict__ fq = a.q + (long long)(q < 2 ? q : q + 1) * fs;  // rho, Vr, Vz, T
   const int ngz = (nz + 15) / 16;
   const long long ngroups = (long long)ngz * nr;
   for (long long grp = (long long)blockIdx.x * INFER_WAVES + wave; grp < ngroups;
@@ -270,65 +285,71 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void mlp_infer_kernel(MlpInfer a)
     }
     {
       const int i = min(i0 + p, nz - 1);
-      for (int n = q; n < MLP_K; n += 4) {
-        float x = 0.0f;
-        if (n < MLP_NIN) x = (mlp_raw_feature(a, i, j, n) - sShift[n]) * sScale[n];
-        act[p][n] = (__bf16)x;
+      int col[5];
+      long long row[5];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) col[c] = min(max(i + c - 2, 0), nz - 1);
+#pragma unroll
+      for (int b = 0; b < 5; ++b) row[b] = (long long)pitch * min(max(j + b - 2, 0), nr - 1);
+      double v[25];
+#pragma unroll
+      for (int b = 0; b < 5; ++b)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) v[5 * b + c] = fq[col[c] + row[b]];
+      const double ctr = v[12];
+#pragma unroll
+      for (int s = 0; s < 25; ++s) {
+        const int n = 25 * q + s;
+        const float x = s == 12 ? (float)ctr : (float)(v[s] - ctr);
+        act[p][n] = (__bf16)((x - sShift[n]) * sScale[n]);
       }
+      const float m = q == 0 ? (float)a.dzl[i] : (q == 1 ? (float)a.drl[j] : (q == 2 ? (float)a.ramp[i] : (float)a.rc[j]));
+      act[p][100 + q] = (__bf16)((m - sShift[100 + q]) * sScale[100 + q]);
+#pragma unroll
+      for (int t = 0; t < 6; ++t) act[p][104 + q + 4 * t] = (__bf16)0.0f;
     }
-    // the activation rows pass between lanes of this wave only: LDS keeps the wave's order,
-    // the barrier keeps the compiler's
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_order();
     for (int layer = 0; layer < 3; ++layer) {
       f32x4 acc[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
       for (int kk = 0; kk < 4; ++kk) {
-        const bf16x8 af = *reinterpret_cast<const bf16x8*>(&act[p][32 * kk + 8 * q]);
+        const bf16x8 xf = *reinterpret_cast<const bf16x8*>(&act[p][32 * kk + 8 * q]);
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-          const bf16x8 bf = *reinterpret_cast<const bf16x8*>(&sW[layer * MLP_H + 16 * t + p][32 * kk + 8 * q]);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf, acc[t], 0, 0, 0);
+          const bf16x8 wf = *reinterpret_cast<const bf16x8*>(&sW[layer * MLP_H + 16 * t + p][32 * kk + 8 * q]);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xf, acc[t], 0, 0, 0);
         }
       }
-      // accumulator v of lane (p, q): point 4 q + v, output 16 t + p
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
-        const int col = 16 * t + p;
-        const float bias = sBias[layer * MLP_H + col];
+        const int o = 16 * t + 4 * q;
+        const f32x4 bias = *reinterpret_cast<const f32x4*>(&sBias[layer * MLP_H + o]);
+        bf16x4 h;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) act[4 * q + v][col] = (__bf16)gelu_f(acc[t][v] + bias);
+        for (int v = 0; v < 4; ++v) h[v] = (__bf16)gelu_f(acc[t][v] + bias[v]);
+        *reinterpret_cast<bf16x4*>(&act[p][o]) = h;
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      wave_order();
     }
     f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
-      const bf16x8 af = *reinterpret_cast<const bf16x8*>(&act[p][32 * kk + 8 * q]);
-      const bf16x8 bf = *reinterpret_cast<const bf16x8*>(&sW[3 * MLP_H + p][32 * kk + 8 * q]);
-      y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf, y, 0, 0, 0);
+      const bf16x8 xf = *reinterpret_cast<const bf16x8*>(&act[p][32 * kk + 8 * q]);
+      const bf16x8 wf = *reinterpret_cast<const bf16x8*>(&sW[3 * MLP_H + p][32 * kk + 8 * q]);
+      y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xf, y, 0, 0, 0);
     }
-    // lanes p = 0, 1 hold beta*, kappa* of points 4 q + v; lane p = 2 clears mu*
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int i = i0 + 4 * q + v;
-      if (i >= nz) continue;
+    // lanes q = 0 hold outputs 0 (beta*) and 1 (kappa*) of point p; mu* is zero
+    const int i = i0 + p;
+    if (q == 0 && i < nz) {
       const long long gi = (long long)i + (long long)pitch * j;
-      if (p < 2) {
-        const float o = y[v] + sBias[3 * MLP_H + p];
-        a.out[(long long)(1 + p) * fs + gi] = a.ramp[i] * (double)a.out_scale[p] * (double)fmaxf(o, 0.0f);
-      } else if (p == 2) {
-        a.out[gi] = 0.0;
-      }
+      const double r = a.ramp[i];
+      a.out[gi] = 0.0;
+      a.out[fs + gi] = r * (double)a.out_scale[0] * (double)fmaxf(y[0] + sBias[3 * MLP_H], 0.0f);
+      a.out[2 * fs + gi] = r * (double)a.out_scale[1] * (double)fmaxf(y[1] + sBias[3 * MLP_H + 1], 0.0f);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_order();
   }
 }
 

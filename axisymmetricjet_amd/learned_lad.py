@@ -1,0 +1,251 @@
+"""Training data, GPU training and scoring of the learned LAD coefficient (ajet_set_learned_lad).
+
+The targets are the reference's beta* and kappa* (ComputeArtTransCoeffs,
+ModShockCapturing.f90:5-106) of states the solver itself generates at jet Mach numbers
+1.3-1.7, divided by the ramp (ModShockCapturing.f90:85), which the network applies exactly.
+Samples are point stencils: half drawn uniformly where the ramp exceeds 0.01, half from the
+points of largest beta* (the shocks the coefficient exists for).  Training runs on the GPU with
+the library's own kernels (ajet_trainer_*).
+
+    python -m axisymmetricjet_amd.learned_lad --out weights.npz
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import time
+
+import numpy as np
+
+from . import capi
+from . import jetcase as jc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_WEIGHTS = os.path.join(HERE, "data", "learned_lad_4400x640.npz")
+MACHS = (1.3, 1.4, 1.5, 1.6, 1.7)
+LAD = dict(shock_capturing=1, C_mu=jc.LAD_C[0], C_beta=jc.LAD_C[1], C_kappa=jc.LAD_C[2])
+FIELDS = [n for n, _ in capi.MLP_FIELDS]
+
+
+def ramp_of(zc):
+    return (1.0 + np.tanh(5.0 * (zc - 10.0))) / 2.0
+
+
+def snapshot_samples(ctx, q, zc, rng, n):
+    """n samples (raw features, targets beta*/ramp and kappa*/ramp) of the state q."""
+    nz, nr = q.shape[:2]
+    _, beta, kappa = ctx.lad_coeffs(q)
+    ramp = ramp_of(zc)
+    valid = np.where(ramp > 0.01)[0]
+    iu = rng.choice(valid, n // 2)
+    ju = rng.integers(0, nr, n // 2)
+    sub = beta[valid[0]:, :]
+    flat = np.argsort(sub.ravel(order="F"))[-max(1, sub.size // 20):]
+    pick = rng.choice(flat, n - n // 2)
+    it = valid[0] + pick % sub.shape[0]
+    jt = pick // sub.shape[0]
+    pts = np.stack([np.concatenate([iu, it]), np.concatenate([ju, jt])], 1).astype(np.int32)
+    x = ctx.lad_features(q, pts)
+    r = ramp[pts[:, 0]]
+    t = np.stack([beta[pts[:, 0], pts[:, 1]] / r, kappa[pts[:, 0], pts[:, 1]] / r], 1).astype(np.float32)
+    return x, t
+
+
+def generate(machs=MACHS, nz=4400, nr=640, warm=200, every=200, nsnap=8, per_snap=4096, seed=7, attempts=4,
+             log=print):
+    """Samples from plain-LAD runs at each Mach number: nsnap snapshots every `every` steps
+    after `warm` steps.  Some noise seeds make the reference's scheme blow up at the corner of
+    the inflow and top boundaries (z = 0, r = r_max) within a few hundred steps; a run whose
+    state turns non-finite or whose |Vr| passes 1 is dropped with its snapshots and the next
+    seed is tried."""
+    rng = np.random.default_rng(seed)
+    xs, ts = [], []
+    for k, mach in enumerate(machs):
+        for a in range(attempts):
+            s0 = seed + 101 * k + 7919 * a
+            case = jc.make_jet(nz, nr, seed=s0, mach=mach)
+            ctx = capi.Context(case, **LAD)
+            got_x, got_t, ok = [], [], True
+            try:
+                ctx.run(warm)
+                for _ in range(nsnap):
+                    ctx.run(every)
+                    q = ctx.state()
+                    if not np.isfinite(q).all() or np.abs(q[:, :, 1]).max() > 1.0:
+                        ok = False
+                        break
+                    x, t = snapshot_samples(ctx, q, case.z[:, 0], rng, per_snap)
+                    got_x.append(x)
+                    got_t.append(t)
+            except capi.AjetError:
+                ok = False
+            ctx.close()
+            if ok:
+                xs += got_x
+                ts += got_t
+                log(f"samples: Mach {mach} (seed {s0}): {nsnap} snapshots to step {warm + nsnap * every}")
+                break
+            log(f"samples: Mach {mach} (seed {s0}): the run left the stable range, dropped")
+    return np.concatenate(xs), np.concatenate(ts)
+
+
+def normalisation(x, t):
+    """shift, scale of the features (mean, 1 / std) and out_scale (RMS of each target)."""
+    shift = x.astype(np.float64).mean(axis=0)
+    std = x.astype(np.float64).std(axis=0)
+    scale = np.where(std > 1e-12 * np.maximum(1.0, np.abs(shift)), 1.0 / np.where(std > 0, std, 1.0), 1.0)
+    out_scale = np.sqrt(np.mean(t.astype(np.float64) ** 2, axis=0))
+    out_scale = np.where(out_scale > 0, out_scale, 1.0)
+    return shift.astype(np.float32), scale.astype(np.float32), out_scale.astype(np.float32)
+
+
+def init_model(seed=0):
+    rng = np.random.default_rng(seed)
+    m = {}
+    dims = [(capi.NHID, capi.NIN), (capi.NHID, capi.NHID), (capi.NHID, capi.NHID), (capi.NOUT, capi.NHID)]
+    for l, (o, i) in enumerate(dims, start=1):
+        m[f"w{l}"] = (rng.standard_normal((o, i)) * np.sqrt(2.0 / i)).astype(np.float32)
+        m[f"b{l}"] = np.zeros(o, np.float32)
+    m["w4"] *= np.float32(0.1)
+    m["shift"] = np.zeros(capi.NIN, np.float32)
+    m["scale"] = np.ones(capi.NIN, np.float32)
+    m["out_scale"] = np.ones(capi.NOUT, np.float32)
+    return m
+
+
+def train(x, t, steps=(3000, 1500, 1500), lrs=(2e-3, 5e-4, 1e-4), batch=4096, seed=0, log=print):
+    """Fit the network to raw features x and targets t on the GPU; returns (model, losses)."""
+    m = init_model(seed)
+    m["shift"], m["scale"], m["out_scale"] = normalisation(x, t)
+    xn = (x.astype(np.float32) - m["shift"]) * m["scale"]
+    tn = (t / m["out_scale"]).astype(np.float32)
+    tr = capi.Trainer(m, batch=batch)
+    tr.load(xn, tn)
+    losses = []
+    for k, (n, lr) in enumerate(zip(steps, lrs)):
+        t0 = time.time()
+        ls = tr.run(n, lr, seed=seed + 17 * k)
+        losses.append(ls)
+        log(f"train: {n} steps at lr {lr:g}: loss {ls[:20].mean():.4e} -> {ls[-20:].mean():.4e} "
+            f"({(time.time() - t0) / n * 1e3:.2f} ms/step)")
+    out = tr.model()
+    for k in ("shift", "scale", "out_scale"):
+        out[k] = m[k]
+    tr.close()
+    return out, np.concatenate(losses)
+
+
+def coefficient_errors(ctx, q, model, ramp_min=1e-8):
+    """Relative L2 errors (over the field) of the network's beta* and kappa* against the
+    reference LAD's for the state q, beside those of zero coefficients (1 by definition)."""
+    _, beta, kappa = ctx.lad_coeffs(q)
+    ctx.set_learned_lad(model, sensor_threshold=1e30, ramp_min=ramp_min)
+    _, bl, kl = ctx.learned_coeffs(q)
+    ctx.set_learned_lad(None, 0.0)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))  # noqa: E731
+    return {"beta": rel(bl, beta), "kappa": rel(kl, kappa)}
+
+
+def zero_model():
+    """A network whose coefficients are zero: the learned mode then keeps only the sensor's
+    fallback blocks -- the method without the network."""
+    m = init_model(0)
+    m["out_scale"] = np.zeros(capi.NOUT, np.float32)
+    return m
+
+
+def score(case, model, sensor_threshold, steps=2000, every=4, ramp_min=1e-8, refresh=1,
+          variants=("learned", "zero")):
+    """Runs of the case under the reference's LAD and under the learned mode (and the
+    zero-coefficient mode), with pressure histories on the arc of probes: OASPL and spectrum
+    differences from the reference run per probe, the compression peaks along the axis and the
+    lip line, the fallback fraction and whether each run stayed finite."""
+    from . import acoustics as ac
+
+    zc, rc = case.z[:, 0], case.r[0, :]
+    pts = ac.arc_points(zc, rc)
+    j_lip = int(np.argmin(np.abs(rc - 0.5)))
+    runs = {}
+    for name in ("lad",) + tuple(variants):
+        ctx = capi.Context(case, **LAD)
+        if name != "lad":
+            ctx.set_learned_lad(model if name == "learned" else zero_model(), sensor_threshold, ramp_min, refresh)
+        ctx.set_probes(pts)
+        rec = {"finite": True}
+        try:
+            t, p = ac.record(ctx, steps, every)
+            q = ctx.state()
+            rec["finite"] = bool(np.isfinite(q).all() and np.isfinite(p).all())
+        except capi.AjetError as e:
+            rec["finite"], rec["error"] = False, str(e)
+        if rec["finite"]:
+            rec.update(t=t, p=p, oaspl=ac.oaspl(p), spec=ac.spectrum(t, p),
+                       shocks={"axis": ac.shock_positions(q, zc, 0), "lip": ac.shock_positions(q, zc, j_lip)})
+            if name != "lad":
+                rec["fallback"] = ctx.fallback_fraction()
+        runs[name] = rec
+        ctx.close()
+    ref = runs["lad"]
+    out = {"steps": steps, "probe_every": every, "probes": len(pts), "sensor_threshold": sensor_threshold,
+           "ramp_min": ramp_min, "refresh": refresh, "reference_finite": ref["finite"]}
+    if not ref["finite"]:
+        return out
+    out["oaspl_lad_db"] = [float(x) for x in ref["oaspl"]]
+    for name in variants:
+        r = runs[name]
+        if not r["finite"]:
+            out[name] = {"finite": False, "error": r.get("error", "non-finite state")}
+            continue
+        d = np.abs(r["oaspl"] - ref["oaspl"])
+        out[name] = {"finite": True, "oaspl_err_db_max": float(d.max()), "oaspl_err_db_mean": float(d.mean()),
+                     "spectrum_err_db": float(np.mean([ac.spectrum_error_db(r["spec"][1][:, k], ref["spec"][1][:, k])
+                                                      for k in range(len(pts))])),
+                     "shock_shift_axis_D": ac.shock_shift(r["shocks"]["axis"], ref["shocks"]["axis"]),
+                     "shock_shift_lip_D": ac.shock_shift(r["shocks"]["lip"], ref["shocks"]["lip"]),
+                     "fallback_fraction": r["fallback"]}
+    return out
+
+
+def save(path, model, **meta):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, **{k: np.asarray(model[k], np.float32) for k in FIELDS},
+             **{f"meta_{k}": np.asarray(v) for k, v in meta.items()})
+
+
+def load(path=DEFAULT_WEIGHTS):
+    with np.load(path, allow_pickle=False) as f:
+        return {k: np.array(f[k], np.float32) for k in FIELDS}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--nz", type=int, default=4400)
+    ap.add_argument("--nr", type=int, default=640)
+    ap.add_argument("--warm", type=int, default=200)
+    ap.add_argument("--every", type=int, default=200)
+    ap.add_argument("--nsnap", type=int, default=8)
+    ap.add_argument("--per-snap", type=int, default=4096)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--steps", type=int, nargs=3, default=(3000, 1500, 1500))
+    ap.add_argument("--seed", type=int, default=7)
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)  # noqa: E731
+    t0 = time.time()
+    x, t = generate(nz=a.nz, nr=a.nr, warm=a.warm, every=a.every, nsnap=a.nsnap, per_snap=a.per_snap,
+                    seed=a.seed, log=log)
+    log(f"samples: {len(x)} in {time.time() - t0:.1f} s")
+    model, losses = train(x, t, steps=tuple(a.steps), batch=a.batch, seed=a.seed, log=log)
+    # held out: the Mach 1.5 run of the benchmark (noise seed 1234, not among the training seeds)
+    case = jc.make_jet(a.nz, a.nr, seed=1234, mach=1.5)
+    ctx = capi.Context(case, **LAD)
+    ctx.run(a.warm + a.nsnap * a.every // 2)
+    err = coefficient_errors(ctx, ctx.state(), model)
+    log(f"held out: relative L2 error beta* {err['beta']:.3f}, kappa* {err['kappa']:.3f} (zero coefficients: 1)")
+    save(a.out, model, samples=len(x), final_loss=float(losses[-50:].mean()), err_beta=err["beta"],
+         err_kappa=err["kappa"], nz=a.nz, nr=a.nr)
+    log(f"wrote {a.out}")
+
+
+if __name__ == "__main__":
+    main()

@@ -192,14 +192,16 @@ typedef struct ajet_mlp {
 } ajet_mlp;
 
 /* Learned mode of a context with shock capturing on (model NULL: back to the reference's LAD).
- * Each step, at its first RK stage, the network gives beta*, kappa* (mu* = 0) at every point
- * where the ramp exceeds ramp_min (zero elsewhere); they are kept for the step's four stages.
+ * At the first RK stage of every refresh-th step (refresh >= 1), the network gives beta*,
+ * kappa* (mu* = 0) at every point of each 16-point group along z whose ramp reaches ramp_min
+ * (zero elsewhere); they are kept until the next refresh.
  * At every stage a dilatation sensor s = max(0, -dil) max(dz, dr) / sqrt(T) is taken, and
  * every z/r filter block (a z segment times an r line group) holding a point with
  * s > sensor_threshold and ramp > ramp_min gets the reference's coefficients of that stage
  * instead: the LAD kernels run only for those blocks and the tiles that feed them.
  * sensor_threshold < 0 reproduces the reference's LAD exactly. */
-int ajet_set_learned_lad(ajet_ctx* ctx, const ajet_mlp* model, double sensor_threshold, double ramp_min);
+int ajet_set_learned_lad(ajet_ctx* ctx, const ajet_mlp* model, double sensor_threshold, double ramp_min,
+                         int32_t refresh);
 
 /* Test and training hooks: the raw features (n x 104, row major) of the points (i, j), 0-based
  * (points holds 2 n int32), of the state q (nz, nr, 5); the network's mu*, beta*, kappa* of
@@ -230,6 +232,12 @@ int ajet_trainer_grads(ajet_trainer* tr, const float* x, const float* t, int32_t
 int ajet_trainer_get(ajet_trainer* tr, ajet_mlp* model);
 int ajet_trainer_destroy(ajet_trainer* tr);
 const char* ajet_trainer_last_error(const ajet_trainer* tr);
+
+/* Probes (an extension: the reference writes whole solution files only): the points (i, j),
+ * 0-based, whose primitives ajet_probe copies out -- values[5 k + v] = pv(i_k, j_k, v) of the
+ * current state -- e.g. pressure histories for far-field acoustics. */
+int ajet_set_probes(ajet_ctx* ctx, const int32_t* points, int32_t n);
+int ajet_probe(ajet_ctx* ctx, double* values);
 
 /* The HIP stream the context launches on (a hipStream_t), for callers that time it. */
 void* ajet_stream(ajet_ctx* ctx);

@@ -190,3 +190,22 @@ def test_sensor_flags_a_compression():
     ctx2.set_learned_lad(rand_model(13, shift_scale=True), sensor_threshold=0.05, ramp_min=1e-8)
     ctx2.run(1)
     assert ctx2.fallback_fraction() < frac
+
+
+def test_refresh_interval():
+    """refresh = k: the network runs at the first step and then every k steps."""
+    case = jc.make_jet(440, 64, seed=1234)
+    m = rand_model(14, shift_scale=True)
+    m["out_scale"] = np.array([2e-4, 2e-5], np.float32)
+    runs = {}
+    for k in (1, 3):
+        c = capi.Context(case, **LAD)
+        c.set_learned_lad(m, sensor_threshold=1e30, ramp_min=1e-8, refresh=k)
+        c.run(1)
+        s1 = c.state()
+        c.run(1)
+        runs[k] = (s1, c.state())
+    assert np.array_equal(runs[1][0], runs[3][0])
+    assert not np.array_equal(runs[1][1], runs[3][1])
+    with pytest.raises(capi.AjetError):
+        capi.Context(case, **LAD).set_learned_lad(m, 1e30, 1e-8, refresh=0)
