@@ -426,8 +426,9 @@ static MlpInfer mlp_args(ajet_ctx* c, const double* q) {
   return m;
 }
 
-// Learned mode: the network's coefficients at the first stage of a step, then at every stage
-// the reference's LAD on the filter blocks the dilatation sensor flags.
+// Learned mode: the network's coefficients at the first stage of every refresh-th step; the
+// dilatation sensor at the first stage of every step; at every stage the reference's LAD on the
+// filter blocks it flagged.
 static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q) {
   hipError_t e;
   if (stage == 1 && c->learned_steps % c->refresh == 0 &&
@@ -436,9 +437,11 @@ static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q) {
   if (stage == 4) ++c->learned_steps;
   LadArgs l = lad_args(c, q);
   l.nseg = c->nseg;
-  if ((e = launch_lad_sensor(l, c->zr_flags, c->line_group, c->sensor_thr, c->ramp_min, c->stream)) != hipSuccess)
-    return e;
-  if ((e = launch_lad_masks(l, c->zr_flags, c->src_mask, c->str_mask, c->stream)) != hipSuccess) return e;
+  if (stage == 1) {
+    if ((e = launch_lad_sensor(l, c->zr_flags, c->line_group, c->sensor_thr, c->ramp_min, c->stream)) != hipSuccess)
+      return e;
+    if ((e = launch_lad_masks(l, c->zr_flags, c->src_mask, c->str_mask, c->stream)) != hipSuccess) return e;
+  }
   l.zr_flags = c->zr_flags;
   l.src_mask = c->src_mask;
   l.str_mask = c->str_mask;

@@ -146,6 +146,11 @@ def coefficient_errors(ctx, q, model, ramp_min=1e-8):
     return {"beta": rel(bl, beta), "kappa": rel(kl, kappa)}
 
 
+def _finite(x):
+    """x, or None where it is not finite (a run with peaks beside one without)."""
+    return float(x) if np.isfinite(x) else None
+
+
 def zero_model():
     """A network whose coefficients are zero: the learned mode then keeps only the sensor's
     fallback blocks -- the method without the network."""
@@ -191,6 +196,7 @@ def score(case, model, sensor_threshold, steps=2000, every=4, ramp_min=1e-8, ref
     if not ref["finite"]:
         return out
     out["oaspl_lad_db"] = [float(x) for x in ref["oaspl"]]
+    out["shocks_lad"] = {"axis": len(ref["shocks"]["axis"]), "lip": len(ref["shocks"]["lip"])}
     for name in variants:
         r = runs[name]
         if not r["finite"]:
@@ -200,8 +206,9 @@ def score(case, model, sensor_threshold, steps=2000, every=4, ramp_min=1e-8, ref
         out[name] = {"finite": True, "oaspl_err_db_max": float(d.max()), "oaspl_err_db_mean": float(d.mean()),
                      "spectrum_err_db": float(np.mean([ac.spectrum_error_db(r["spec"][1][:, k], ref["spec"][1][:, k])
                                                       for k in range(len(pts))])),
-                     "shock_shift_axis_D": ac.shock_shift(r["shocks"]["axis"], ref["shocks"]["axis"]),
-                     "shock_shift_lip_D": ac.shock_shift(r["shocks"]["lip"], ref["shocks"]["lip"]),
+                     "shock_shift_axis_D": _finite(ac.shock_shift(r["shocks"]["axis"], ref["shocks"]["axis"])),
+                     "shock_shift_lip_D": _finite(ac.shock_shift(r["shocks"]["lip"], ref["shocks"]["lip"])),
+                     "shocks_axis": len(r["shocks"]["axis"]), "shocks_lip": len(r["shocks"]["lip"]),
                      "fallback_fraction": r["fallback"]}
     return out
 

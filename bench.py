@@ -20,6 +20,13 @@ Prints one JSON line.
              command, kept in MEASURED_TRAFFIC
   cpu_baseline  the C restatement (oracle) of the same step on the host cores, OpenMP over
              grid lines, on a bounded sample of steps of the same grid
+  learned    the same jet with the learned LAD coefficient (ajet_set_learned_lad, the weights in
+             axisymmetricjet_amd/data): updates/s and the speed-up over the reference's LAD, the
+             inference kernel against the BF16 MFMA peak (92.7 kFLOP per point evaluated:
+             2 (104 x 128 + 2 x 128 x 128 + 128 x 2)), and a scoring run from the same initial
+             condition -- far-field OASPL and spectrum on an arc of probes, compression peaks on
+             the axis and lip line -- against the reference-LAD run and against the learned mode
+             without the network (zero coefficients outside the sensor's blocks)
 """
 from __future__ import annotations
 
@@ -34,6 +41,8 @@ sys.path.insert(0, ROOT)
 
 BYTES_PER_UPDATE = 640
 HBM_PEAK_GBPS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0   # dense MFMA
+MLP_FLOP_PER_POINT = 2 * (104 * 128 + 2 * 128 * 128 + 128 * 2)
 
 # HBM bytes per grid-point update of a whole step, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE over the same workload (FETCH scaled by 2.0 and WRITE by 1.0, the factors a 1 GiB
@@ -76,6 +85,12 @@ def main(argv=None):
     ap.add_argument("--ensemble-steps", type=int, default=400,
                     help="the ensemble's runs diverge after several hundred steps under the reference's "
                          "numerics, so 400 is the default rather than 2000")
+    ap.add_argument("--learned-steps", type=int, default=1000, help="steps of the learned-LAD mode timed (0: skip)")
+    ap.add_argument("--learned-refresh", type=int, default=4, help="the network runs every this many steps")
+    ap.add_argument("--sensor-threshold", type=float, default=0.1)
+    ap.add_argument("--ramp-min", type=float, default=1e-8)
+    ap.add_argument("--score-steps", type=int, default=5000, help="steps of the learned-LAD scoring runs (0: skip)")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "axisymmetricjet_amd", "data", "learned_lad_4400x640.npz"))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -160,6 +175,61 @@ def main(argv=None):
     achieved = BYTES_PER_UPDATE * npts / (step_kernel_ms / 1e3) / 1e9
     log("stage avg ms: " + ", ".join(f"{x:.4f}" for x in stage_ms))
 
+    learned = None
+    lad_ms = ms / args.steps if args.method == "lad" else others.get("lad", {}).get("ms_per_step")
+    if args.learned_steps > 0 and os.path.exists(args.weights):
+        from axisymmetricjet_amd import learned_lad as ll
+        model = ll.load(args.weights)
+        lctx = context("lad")
+        lctx.set_learned_lad(model, args.sensor_threshold, args.ramp_min, args.learned_refresh)
+        lms, _ = timed_run(lctx, args.learned_steps, args.warmup)
+        frac = lctx.fallback_fraction()
+        lctx.close()
+
+        # the inference kernel: stage 1 with the network (refresh 1) minus stage 1 without it
+        def stage1_ms(refresh, n=10):
+            c = context("lad")
+            c.set_learned_lad(model, args.sensor_threshold, args.ramp_min, refresh)
+            c.run(1)
+            st = torch.cuda.ExternalStream(c.stream())
+            tot = 0.0
+            for _ in range(n):
+                c.compute_dt(sync=False)
+                a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a0.record(st)
+                c.rk4_stage(1)
+                a1.record(st)
+                for s in (2, 3, 4):
+                    c.rk4_stage(s)
+                torch.cuda.synchronize()
+                tot += a0.elapsed_time(a1)
+            c.close()
+            return tot / n
+        mlp_ms = stage1_ms(1) - stage1_ms(1 << 30)
+        import numpy as np
+        zc = case.z[:, 0]
+        ramp = (1.0 + np.tanh(5.0 * (zc - 10.0))) / 2.0
+        last = np.minimum(np.arange(0, nz, 16) + 15, nz - 1)
+        live = sum(min(16, nz - i0) for i0, il in zip(range(0, nz, 16), last) if ramp[il] >= args.ramp_min) * nr
+        tflops = MLP_FLOP_PER_POINT * live / (mlp_ms / 1e3) / 1e12
+        learned = {"value": npts * args.learned_steps / (lms / 1e3), "unit": "grid-point updates/s",
+                   "ms_per_step": lms / args.learned_steps, "steps": args.learned_steps,
+                   "speedup_vs_lad": (lad_ms / (lms / args.learned_steps)) if lad_ms else None,
+                   "refresh": args.learned_refresh, "sensor_threshold": args.sensor_threshold,
+                   "ramp_min": args.ramp_min, "fallback_fraction": frac, "weights": os.path.basename(args.weights),
+                   "network": "104-128-128-128-2, GELU, BF16 weights and activations, FP32 accumulation",
+                   "inference_roofline": {"bound": "mfma", "achieved": tflops, "peak": BF16_PEAK_TFLOPS,
+                                          "unit": "TFLOP/s", "frac": tflops / BF16_PEAK_TFLOPS,
+                                          "kernel_ms": mlp_ms, "points": live,
+                                          "flop_per_point": MLP_FLOP_PER_POINT}}
+        log(f"learned: {learned['ms_per_step']:.4f} ms/step, x{learned['speedup_vs_lad'] or 0:.3f} vs LAD, "
+            f"inference {mlp_ms:.4f} ms ({tflops:.1f} TFLOP/s), fallback {frac:.4f}")
+        if args.score_steps > 0:
+            t0 = time.time()
+            learned["score"] = ll.score(case, model, args.sensor_threshold, steps=args.score_steps, every=4,
+                                        ramp_min=args.ramp_min, refresh=args.learned_refresh)
+            log(f"learned: scoring runs of {args.score_steps} steps in {time.time() - t0:.1f} s")
+
     ensemble = None
     if args.ensemble_runs > 0 and args.ensemble_steps > 0:
         ctx.close()
@@ -227,6 +297,7 @@ def main(argv=None):
                      "kernel": "one RK stage: artificial-diffusivity launches (lad) + stage_kernel<stage>",
                      "stage_ms": stage_ms, "bytes_per_update": BYTES_PER_UPDATE},
         "other_methods": others,
+        "learned": learned,
         "ensemble": ensemble,
         "cpu_baseline": cpu,
     }

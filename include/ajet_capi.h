@@ -194,19 +194,19 @@ typedef struct ajet_mlp {
 /* Learned mode of a context with shock capturing on (model NULL: back to the reference's LAD).
  * At the first RK stage of every refresh-th step (refresh >= 1), the network gives beta*,
  * kappa* (mu* = 0) at every point of each 16-point group along z whose ramp reaches ramp_min
- * (zero elsewhere); they are kept until the next refresh.
- * At every stage a dilatation sensor s = max(0, -dil) max(dz, dr) / sqrt(T) is taken, and
- * every z/r filter block (a z segment times an r line group) holding a point with
- * s > sensor_threshold and ramp > ramp_min gets the reference's coefficients of that stage
- * instead: the LAD kernels run only for those blocks and the tiles that feed them.
- * sensor_threshold < 0 reproduces the reference's LAD exactly. */
+ * (zero elsewhere); they are kept until the next refresh.  At the first stage of every step a
+ * dilatation sensor s = max(0, -dil) max(dz, dr) / sqrt(T) is taken, and every z/r filter block
+ * (a z segment times an r line group) holding a point with s > sensor_threshold and
+ * ramp > ramp_min gets the reference's coefficients at each of the step's four stages instead:
+ * the LAD kernels run only for those blocks and the tiles that feed them.  sensor_threshold < 0
+ * reproduces the reference's LAD exactly. */
 int ajet_set_learned_lad(ajet_ctx* ctx, const ajet_mlp* model, double sensor_threshold, double ramp_min,
                          int32_t refresh);
 
 /* Test and training hooks: the raw features (n x 104, row major) of the points (i, j), 0-based
  * (points holds 2 n int32), of the state q (nz, nr, 5); the network's mu*, beta*, kappa* of
  * the state q (no fallback), each (nz, nr); the fraction of filter blocks the sensor sent to
- * the reference's LAD in the last stage taken. */
+ * the reference's LAD in the last step taken. */
 int ajet_lad_features(ajet_ctx* ctx, const double* q, const int32_t* points, int32_t n, float* x);
 int ajet_lad_learned_coeffs(ajet_ctx* ctx, const double* q, double* mustar, double* betastar,
                             double* kappastar);
