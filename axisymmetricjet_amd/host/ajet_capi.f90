@@ -28,6 +28,18 @@ MODULE ajet_capi
     REAL(C_DOUBLE) :: amp, strouhal, uj, t0
   END TYPE ajet_forcing_t
 
+  ! ajet_mlp: the learned LAD network (include/ajet_capi.h).  The C arrays w1[128][104] etc. are
+  ! row major, so their Fortran mirrors take the indices in the other order: w1(k, o) is the
+  ! weight of input k in output o.
+  TYPE, BIND(C) :: ajet_mlp_t
+    REAL(C_FLOAT) :: w1(104, 128), b1(128)
+    REAL(C_FLOAT) :: w2(128, 128), b2(128)
+    REAL(C_FLOAT) :: w3(128, 128), b3(128)
+    REAL(C_FLOAT) :: w4(128, 2), b4(2)
+    REAL(C_FLOAT) :: shift(104), scale(104)
+    REAL(C_FLOAT) :: out_scale(2)
+  END TYPE ajet_mlp_t
+
   INTERFACE
     SUBROUTINE ajet_params_default(p, nz, nr) BIND(C, name='ajet_params_default')
       IMPORT :: ajet_params_t, C_INT32_T
@@ -83,6 +95,15 @@ MODULE ajet_capi
       TYPE(C_PTR), VALUE :: ctx
       TYPE(ajet_forcing_t), INTENT(IN) :: f
     END FUNCTION ajet_set_forcing
+
+    INTEGER(C_INT) FUNCTION ajet_set_learned_lad(ctx, model, sensor_threshold, ramp_min, refresh) &
+        BIND(C, name='ajet_set_learned_lad')
+      IMPORT :: C_INT, C_PTR, C_DOUBLE, C_INT32_T, ajet_mlp_t
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(ajet_mlp_t), INTENT(IN) :: model
+      REAL(C_DOUBLE), VALUE :: sensor_threshold, ramp_min
+      INTEGER(C_INT32_T), VALUE :: refresh
+    END FUNCTION ajet_set_learned_lad
 
     INTEGER(C_INT) FUNCTION ajet_get_dt_log(ctx, dts, n, time) BIND(C, name='ajet_get_dt_log')
       IMPORT :: C_INT, C_PTR, C_INT32_T

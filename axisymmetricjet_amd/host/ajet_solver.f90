@@ -23,6 +23,11 @@ MODULE ModuleVariables
   ! ControlForcing = 1 adds the feedback forcing of ModControlForcing.f90 read from forcing_file
   INTEGER :: ControlForcing = 0
   CHARACTER(LEN=256) :: forcing_file = 'Vz_Control_Vr_Feedback.dat'
+  ! LearnedLAD = 1: the learned LAD coefficient (ajet_set_learned_lad) with the network of
+  ! mlp_file, a raw little-endian copy of the ajet_mlp struct
+  INTEGER :: LearnedLAD = 0, mlp_refresh = 4
+  CHARACTER(LEN=256) :: mlp_file = 'learned_lad.bin'
+  REAL(REAL64) :: sensor_threshold = 0.1D0, ramp_min = 1.0D-8
   INTEGER :: noutput = 200
   INTEGER(INT64) :: no_iterations = 10000
   INTEGER(INT64) :: iteration, iteration_start
@@ -140,6 +145,23 @@ CONTAINS
     f%t0 = 0.0D0
   END SUBROUTINE ForcingRead
 
+  ! The network of the learned LAD coefficient from mlp_file (the bytes of one ajet_mlp).
+  SUBROUTINE MlpRead(m)
+    TYPE(ajet_mlp_t), INTENT(OUT) :: m
+    INTEGER :: u, ios
+    OPEN (NEWUNIT=u, FILE=TRIM(mlp_file), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='old', IOSTAT=ios)
+    IF (ios /= 0) THEN
+      PRINT *, 'cannot open ', TRIM(mlp_file)
+      CALL StopTheCode
+    END IF
+    READ (u, IOSTAT=ios) m
+    IF (ios /= 0) THEN
+      PRINT *, 'cannot read the network from ', TRIM(mlp_file)
+      CALL StopTheCode
+    END IF
+    CLOSE (u)
+  END SUBROUTINE MlpRead
+
   SUBROUTINE fail_size(f)
     CHARACTER(LEN=*), INTENT(IN) :: f
     PRINT *, 'solution size differs from the grid: ', TRIM(f)
@@ -226,9 +248,11 @@ PROGRAM ModSolver
   INTEGER :: u, ios
   TYPE(ajet_params_t) :: p
   TYPE(ajet_forcing_t) :: fc
+  TYPE(ajet_mlp_t) :: mlp
   NAMELIST /ajet_input/ GRID_FILENAME, INITIAL_CONDITION, TARGET_FILE, bc_file, out_prefix, Re, Pr, CFL, &
                         ShockCapturing, C_mu, C_beta, C_kappa, do_viscous_terms, no_iterations, noutput, &
-                        ControlForcing, forcing_file
+                        ControlForcing, forcing_file, LearnedLAD, mlp_file, sensor_threshold, ramp_min, &
+                        mlp_refresh
 
   ! run-time inputs the reference compiles in (ModuleVariables.f90:6-9, 69, 73, 152, 164, 177;
   ! ModSolver.f90:116), read from a namelist file (default ajet.nml) when present
@@ -260,6 +284,11 @@ PROGRAM ModSolver
   IF (ControlForcing == 1) THEN
     CALL ForcingRead(fc)
     CALL check(ajet_set_forcing(ctx, fc), 'ajet_set_forcing')
+  END IF
+  IF (LearnedLAD == 1) THEN
+    CALL MlpRead(mlp)
+    CALL check(ajet_set_learned_lad(ctx, mlp, sensor_threshold, ramp_min, INT(mlp_refresh, C_INT32_T)), &
+               'ajet_set_learned_lad')
   END IF
   CALL SolnRead
   CALL check(ajet_set_state(ctx, pv, pv_target), 'ajet_set_state')

@@ -219,6 +219,14 @@ def save(path, model, **meta):
              **{f"meta_{k}": np.asarray(v) for k, v in meta.items()})
 
 
+def export_raw(model, path):
+    """The bytes of one ajet_mlp (little-endian float32, the C struct layout): the network file
+    the Fortran host reads (its namelist's mlp_file)."""
+    m = capi.mlp_to_c(model)
+    with open(path, "wb") as f:
+        f.write(bytes(m))
+
+
 def load(path=DEFAULT_WEIGHTS):
     with np.load(path, allow_pickle=False) as f:
         return {k: np.array(f[k], np.float32) for k in FIELDS}

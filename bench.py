@@ -47,7 +47,7 @@ MLP_FLOP_PER_POINT = 2 * (104 * 128 + 2 * 128 * 128 + 128 * 2)
 # HBM bytes per grid-point update of a whole step, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE over the same workload (FETCH scaled by 2.0 and WRITE by 1.0, the factors a 1 GiB
 # device copy reads on the same counters); None where not measured.
-MEASURED_TRAFFIC = {"plain": 723.1, "lad": 2030.6}  # profiles/step_jet4400x640_{plain,lad}.md
+MEASURED_TRAFFIC = {"plain": 723.1, "lad": 1712.4, "learned": 913.9}  # profiles/step_jet4400x640_{plain,lad}.md, learned_lad_jet4400x640.md
 
 
 def log(msg):
@@ -77,8 +77,9 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nz", type=int, default=4400)
     ap.add_argument("--nr", type=int, default=640)
-    ap.add_argument("--method", choices=["lad", "plain"], default="lad",
-                    help="lad: ShockCapturing = 1 with the LAD constants; plain: ShockCapturing = 0")
+    ap.add_argument("--method", choices=["lad", "plain", "learned"], default="lad",
+                    help="lad: ShockCapturing = 1 with the LAD constants; plain: ShockCapturing = 0; learned: "
+                         "lad with the learned coefficient (the --learned-* options)")
     ap.add_argument("--other-steps", type=int, default=200, help="steps of the other method timed beside")
     ap.add_argument("--stage-steps", type=int, default=20, help="steps timed launch by launch for the roofline")
     ap.add_argument("--ensemble-runs", type=int, default=64, help="runs of the batched ensemble (0: skip it)")
@@ -121,9 +122,13 @@ def main(argv=None):
     case = jc.make_jet(nz, nr, seed=1234)
 
     def context(method):
-        if method == "lad":
+        if method in ("lad", "learned"):
             c = jc.LAD_C
-            return capi.Context(case, shock_capturing=1, C_mu=c[0], C_beta=c[1], C_kappa=c[2])
+            ctx = capi.Context(case, shock_capturing=1, C_mu=c[0], C_beta=c[1], C_kappa=c[2])
+            if method == "learned":
+                from axisymmetricjet_amd import learned_lad as ll
+                ctx.set_learned_lad(ll.load(args.weights), args.sensor_threshold, args.ramp_min, args.learned_refresh)
+            return ctx
         return capi.Context(case)
 
     def timed_run(ctx, steps, warmup):
@@ -142,7 +147,7 @@ def main(argv=None):
             raise RuntimeError(f"implausible time step {dts[-1]}")
         return ms, dts[-1]
 
-    other = "plain" if args.method == "lad" else "lad"
+    other = "plain" if args.method != "plain" else "lad"
     others = {}
     if args.other_steps > 0:
         octx = context(other)
@@ -267,9 +272,9 @@ def main(argv=None):
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         log(f"cpu baseline: oracle, {threads} threads, ~{args.cpu_seconds:.0f} s")
-        v, n, el = cpu_baseline(case, args.cpu_seconds, threads, jc.LAD_C if args.method == "lad" else None)
+        v, n, el = cpu_baseline(case, args.cpu_seconds, threads, jc.LAD_C if args.method != "plain" else None)
         cpu = {"value": v, "unit": "grid-point updates/s", "cores": threads, "kind": "port",
-               "sample": f"{n} RK4 steps ({args.method}) of the same {nz}x{nr} jet (after 1 warm-up step), "
+               "sample": f"{n} RK4 steps ({'lad' if args.method == 'learned' else args.method}) of the same {nz}x{nr} jet (after 1 warm-up step), "
                          f"{el:.1f} s, C restatement at -O2 without FMA, OpenMP over grid lines"}
         log(f"cpu baseline {v:.4e} updates/s")
 
@@ -289,7 +294,8 @@ def main(argv=None):
         "config": {"workload": f"FP64 axisymmetric jet RK4 step, {nz}x{nr} (z x r), Mach 1.5, Re 1e4, Pr 0.7, "
                                f"CFL 0.5, SAT/NSCBC/axis boundaries, "
                                + ("shock capturing and filtering on (LAD C = %g, %g, %g)" % jc.LAD_C
-                                  if args.method == "lad" else "shock capturing off"),
+                                  if args.method != "plain" else "shock capturing off")
+                               + (", learned LAD coefficient" if args.method == "learned" else ""),
                    "nz": nz, "nr": nr, "points": npts, "method": args.method},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
