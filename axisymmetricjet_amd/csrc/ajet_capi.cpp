@@ -429,32 +429,33 @@ static MlpInfer mlp_args(ajet_ctx* c, const double* q) {
 // Learned mode: the network's coefficients at the first stage of every refresh-th step; the
 // dilatation sensor at the first stage of every step; at every stage the reference's LAD on the
 // filter blocks it flagged.
-static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q) {
+static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStream_t s) {
   hipError_t e;
   if (stage == 1 && c->learned_steps % c->refresh == 0 &&
-      (e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, c->stream)) != hipSuccess)
+      (e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, s)) != hipSuccess)
     return e;
   if (stage == 4) ++c->learned_steps;
   LadArgs l = lad_args(c, q);
   l.nseg = c->nseg;
   if (stage == 1) {
-    if ((e = launch_lad_sensor(l, c->zr_flags, c->line_group, c->sensor_thr, c->ramp_min, c->stream)) != hipSuccess)
-      return e;
-    if ((e = launch_lad_masks(l, c->zr_flags, c->src_mask, c->str_mask, c->stream)) != hipSuccess) return e;
+    if ((e = launch_lad_sensor(l, c->zr_flags, c->line_group, c->sensor_thr, c->ramp_min, s)) != hipSuccess) return e;
+    if ((e = launch_lad_masks(l, c->zr_flags, c->src_mask, c->str_mask, s)) != hipSuccess) return e;
   }
   l.zr_flags = c->zr_flags;
   l.src_mask = c->src_mask;
   l.str_mask = c->str_mask;
-  return launch_lad(l, c->stream);
+  return launch_lad(l, s);
+}
+
+// The artificial coefficients of one stage of a run in LAD mode (the reference's or learned).
+static hipError_t launch_coefficients(ajet_ctx* c, int stage, const double* q, hipStream_t s) {
+  return c->learned ? launch_learned(c, stage, q, s) : launch_lad(lad_args(c, q), s);
 }
 
 static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
   const int mode = physics_mode(c->p);
   hipError_t e;
-  if (mode == MODE_LAD) {
-    e = c->learned ? launch_learned(c, stage, a.qin) : launch_lad(lad_args(c, a.qin), c->stream);
-    if (e != hipSuccess) return e;
-  }
+  if (mode == MODE_LAD && (e = launch_coefficients(c, stage, a.qin, c->stream)) != hipSuccess) return e;
   if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
   a.tiles = c->bnd_tiles;
@@ -794,9 +795,10 @@ int ajet_batch_create(ajet_ctx* const* runs, int32_t nruns, ajet_batch** out) {
     if (!c || !c->stream) return bfail(b, AJET_EINVAL, "null context");
     if (c->device != runs[0]->device) return bfail(b, AJET_EINVAL, "runs on different devices");
     const int m = physics_mode(c->p);
-    if (m != MODE_PLAIN && m != MODE_INVISCID)
-      return bfail(b, AJET_EUNSUPPORTED, "batched runs need shock capturing off");
-    if (m != physics_mode(runs[0]->p)) return bfail(b, AJET_EINVAL, "runs differ in do_viscous_terms");
+    if (m == MODE_SC)
+      return bfail(b, AJET_EUNSUPPORTED, "batched shock capturing needs non-zero C_mu, C_beta, C_kappa");
+    if (m != physics_mode(runs[0]->p))
+      return bfail(b, AJET_EINVAL, "runs differ in do_viscous_terms or shock capturing");
     if (c->p.forcing && !c->forcing_set) return bfail(b, AJET_EINVAL, "a run's forcing parameters are not set");
     if ((int)c->s.zcuts.size() * (int)c->s.rcuts.size() >= (1 << 20)) return bfail(b, AJET_EINVAL, "grid too large");
     b->runs.push_back(c);
@@ -853,9 +855,17 @@ int ajet_batch_create(ajet_ctx* const* runs, int32_t nruns, ajet_batch** out) {
   return AJET_OK;
 }
 
-// One RK stage of every run: boundary tiles on the second stream beside the interior tiles.
+// One RK stage of every run: the artificial coefficients of each LAD run (its own launches, in
+// run order), then the stage kernels of all runs, boundary tiles on the second stream beside
+// the interior tiles.
 static hipError_t batch_stage(ajet_batch* b, int stage, int dt_from_dmax, int parity) {
   hipError_t e;
+  if (b->mode == MODE_LAD) {
+    for (ajet_ctx* c : b->runs) {
+      const double* q = stage == 1 ? c->Q0 : (stage == 3 ? c->Q2 : c->Q1);
+      if ((e = launch_coefficients(c, stage, q, b->stream)) != hipSuccess) return e;
+    }
+  }
   if ((e = hipEventRecord(b->ev_fork, b->stream)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(b->aux, b->ev_fork, 0)) != hipSuccess) return e;
   if ((e = launch_stage_batch(stage, b->mode, false, b->force, b->args[stage - 1], b->d_bnd, b->n_bnd, dt_from_dmax,
@@ -903,8 +913,15 @@ int ajet_batch_run(ajet_batch* b, int32_t nsteps) {
   int parity = parity0;
   for (int st = 1; st <= 4; ++st) BHIPCHK(b, batch_stage(b, st, 0, parity));
   parity ^= 1;
-  // later steps: stage 1 finishes the previous step's fused reduction; one graph per parity
+  // later steps: stage 1 finishes the previous step's fused reduction; one graph per parity,
+  // except with LAD runs (launched directly: the learned mode's network runs every refresh-th
+  // step, and each stage's LAD work dwarfs its launch cost)
   for (int k = 1; k < nsteps; ++k) {
+    if (b->mode == MODE_LAD) {
+      for (int st = 1; st <= 4; ++st) BHIPCHK(b, batch_stage(b, st, 1, parity));
+      parity ^= 1;
+      continue;
+    }
     if (!b->graph[parity]) {
       hipGraph_t g = nullptr;
       BHIPCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));

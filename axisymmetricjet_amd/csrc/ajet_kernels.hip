@@ -535,6 +535,11 @@ hipError_t launch_stage_batch(int stage, int mode, bool interior, bool force, co
                         : launch_batch_t<MODE_INVISCID, true, false>(stage, grid, runs, list, n, dt_from_dmax, parity, s);
     else force ? launch_batch_t<MODE_INVISCID, false, true>(stage, grid, runs, list, n, dt_from_dmax, parity, s)
                : launch_batch_t<MODE_INVISCID, false, false>(stage, grid, runs, list, n, dt_from_dmax, parity, s);
+  } else if (mode == MODE_LAD) {
+    if (interior) force ? launch_batch_t<MODE_LAD, true, true>(stage, grid, runs, list, n, dt_from_dmax, parity, s)
+                        : launch_batch_t<MODE_LAD, true, false>(stage, grid, runs, list, n, dt_from_dmax, parity, s);
+    else force ? launch_batch_t<MODE_LAD, false, true>(stage, grid, runs, list, n, dt_from_dmax, parity, s)
+               : launch_batch_t<MODE_LAD, false, false>(stage, grid, runs, list, n, dt_from_dmax, parity, s);
   } else {
     return hipErrorInvalidValue;
   }

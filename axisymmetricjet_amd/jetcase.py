@@ -90,6 +90,16 @@ def ensemble(n: int = 64, seed: int = 2026):
     return out
 
 
+def combined_specs(n: int = 32, seed: int = 4242):
+    """The seeded forcing sweep of the combined learned-LAD form, as parameters (the 4400 x 640
+    cases are large, so the caller builds them one at a time): jet Mach uniform in [1.4, 1.6],
+    Strouhal number in [0.2, 0.8], excitation amplitude in [0.5%, 3%] of Uj, noise seed per
+    member.  Returns [(mach, strouhal, amp, noise_seed), ...]."""
+    rng = np.random.default_rng(seed)
+    return [(float(rng.uniform(1.4, 1.6)), float(rng.uniform(0.2, 0.8)), float(rng.uniform(0.005, 0.03)),
+             seed * 1000 + k) for k in range(n)]
+
+
 # LAD constants C_mu, C_beta, C_kappa of the shock-capturing runs (the reference ships 0, 0, 0).
 # C_beta is a tenth of the usual 1.75: the reference's time step ignores the artificial
 # viscosities, and with 1.75 the seeded grid-scale noise drives beta* to ~70 at the outflow

@@ -92,6 +92,11 @@ def main(argv=None):
     ap.add_argument("--ramp-min", type=float, default=1e-8)
     ap.add_argument("--score-steps", type=int, default=5000, help="steps of the learned-LAD scoring runs (0: skip)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "axisymmetricjet_amd", "data", "learned_lad_4400x640.npz"))
+    ap.add_argument("--combined-runs", type=int, default=32,
+                    help="runs of the combined form: learned-LAD jets at the bench grid with a forcing sweep (0: skip)")
+    ap.add_argument("--combined-steps", type=int, default=50, help="steps of the combined batch timed")
+    ap.add_argument("--combined-score-steps", type=int, default=400,
+                    help="steps of the combined batch's OASPL comparison with the same batch under the reference LAD")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -268,6 +273,79 @@ def main(argv=None):
         for c in ectx:
             c.close()
 
+    combined = None
+    if args.combined_runs > 0 and args.combined_steps > 0 and os.path.exists(args.weights):
+        import numpy as np
+        from axisymmetricjet_amd import acoustics as ac
+        from axisymmetricjet_amd import learned_lad as ll
+        ctx.close()  # the main context's buffers are not needed any more
+        model = ll.load(args.weights)
+        specs = jc.combined_specs(args.combined_runs)
+        pts = ac.arc_points(case.z[:, 0], case.r[0, :])
+        c = jc.LAD_C
+
+        def combined_batch(learned):
+            runs = []
+            for mach, st, amp, seed in specs:
+                cs = jc.make_jet(nz, nr, seed=seed, mach=mach)
+                r = capi.Context(cs, forcing=jc.excitation(mach, strouhal=st, amp=amp), shock_capturing=1,
+                                 C_mu=c[0], C_beta=c[1], C_kappa=c[2])
+                if learned:
+                    r.set_learned_lad(model, args.sensor_threshold, args.ramp_min, args.learned_refresh)
+                r.set_probes(pts)
+                runs.append(r)
+                del cs
+            return runs, capi.Batch(runs)
+
+        def timed_batch(runs, b, steps):
+            bstream = torch.cuda.ExternalStream(runs[0].stream())
+            b.run(args.warmup)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(bstream)
+            b.run(steps)
+            e1.record(bstream)
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1)
+
+        def probe_history(runs, b, steps, every=8):
+            ps = []
+            for _ in range(steps // every):
+                b.run(every)
+                vs = [r.probe() for r in runs]
+                ps.append([v[:, 0] * v[:, 4] for v in vs])
+            return np.array(ps)  # (samples, runs, probes)
+
+        cpts = nz * nr * len(specs)
+        combined = {"runs": len(specs), "points": cpts, "steps": args.combined_steps,
+                    "workload": f"{len(specs)} jets of {nz}x{nr}, Mach 1.4-1.6, inflow excitation St 0.2-0.8, "
+                                "amplitude 0.5-3% Uj, seeded noise; each run's coefficients its own launches, "
+                                "stage kernels batched over all runs"}
+        hist = {}
+        for name in ("learned", "lad"):
+            t0 = time.time()
+            runs, b = combined_batch(name == "learned")
+            log(f"combined ({name}): {len(runs)} runs built in {time.time() - t0:.1f} s")
+            bms = timed_batch(runs, b, args.combined_steps)
+            combined[name] = {"value": cpts * args.combined_steps / (bms / 1e3), "ms_per_step": bms / args.combined_steps}
+            for r in runs:
+                d, _ = r.dt_log()
+                if not (d[-1] > 0 and d[-1] < 1.0):
+                    raise RuntimeError(f"combined: implausible time step {d[-1]}")
+            if args.combined_score_steps > 0:
+                hist[name] = probe_history(runs, b, args.combined_score_steps)
+            log(f"combined ({name}): {combined[name]['ms_per_step']:.3f} ms/step, {combined[name]['value']:.4e} updates/s")
+            b.close()
+            for r in runs:
+                r.close()
+        combined["speedup_vs_lad"] = combined["lad"]["ms_per_step"] / combined["learned"]["ms_per_step"]
+        if hist:
+            oa = {k: np.stack([ac.oaspl(v[:, r, :]) for r in range(v.shape[1])]) for k, v in hist.items()}
+            d = np.abs(oa["learned"] - oa["lad"])
+            combined["score"] = {"steps": args.combined_score_steps, "probe_every": 8, "probes": len(pts),
+                                 "oaspl_err_db_max": float(d.max()), "oaspl_err_db_mean": float(d.mean()),
+                                 "finite": bool(np.isfinite(hist["learned"]).all() and np.isfinite(hist["lad"]).all())}
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -305,6 +383,7 @@ def main(argv=None):
         "other_methods": others,
         "learned": learned,
         "ensemble": ensemble,
+        "combined": combined,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -161,10 +161,12 @@ int ajet_set_forcing(ajet_ctx* ctx, const ajet_forcing* f);
  * tiles of all runs, and the steps after the first replay a captured HIP graph.  Each run keeps
  * its own grid, state, time step, time, forcing and time-step log, and ends exactly as
  * ajet_run on that context alone would leave it (bit for bit).  All contexts on one device,
- * shock capturing off (the LAD kernels are not batched), the same do_viscous_terms, states
- * loaded and, with forcing, its parameters set; all must have taken the same number of steps
- * modulo 2.  The contexts must outlive the batch; between ajet_batch_run calls they can be used
- * on their own. */
+ * the same do_viscous_terms and shock capturing, states loaded and, with forcing, its
+ * parameters set; all must have taken the same number of steps modulo 2.  With shock capturing
+ * (non-zero C_mu, C_beta, C_kappa) each run's artificial coefficients -- the reference's LAD or
+ * the learned ones of ajet_set_learned_lad -- are its own launches before each batched stage,
+ * and steps are launched directly rather than replayed.  The contexts must outlive the batch;
+ * between ajet_batch_run calls they can be used on their own. */
 typedef struct ajet_batch ajet_batch;
 int ajet_batch_create(ajet_ctx* const* runs, int32_t nruns, ajet_batch** out);
 int ajet_batch_run(ajet_batch* b, int32_t nsteps);

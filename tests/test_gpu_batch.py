@@ -52,14 +52,51 @@ def test_batch_matches_runs_alone_bit_for_bit():
     b.close()
 
 
-def test_batch_rejects_shock_capturing_runs():
+def test_batch_rejects_mixed_or_zero_coefficient_shock_capturing():
     case = jc.make_jet(64, 32, seed=2)
     a = capi.Context(case)
     c = gc.LAD_C
     l = capi.Context(case, shock_capturing=1, C_mu=c[0], C_beta=c[1], C_kappa=c[2])
     with pytest.raises(capi.AjetError) as ei:
         capi.Batch([a, l])
+    assert ei.value.code == -1
+    z = capi.Context(case, shock_capturing=1)
+    with pytest.raises(capi.AjetError) as ei:
+        capi.Batch([z])
     assert ei.value.code == -8
+
+
+def test_lad_and_learned_batch_matches_runs_alone_bit_for_bit():
+    """Runs with the reference's LAD and with the learned coefficient (and forcing) in one
+    batch: each run's coefficients are its own launches, the stage kernels are batched."""
+    from axisymmetricjet_amd import learned_lad as ll
+
+    c = gc.LAD_C
+    model = ll.load()
+
+    def make():
+        out = []
+        specs = [(jc.make_jet(440, 64, seed=41), None, None),
+                 (jc.make_jet(520, 80, seed=42, mach=1.4), jc.excitation(1.4, strouhal=0.4, amp=0.02), 2),
+                 (jc.make_jet(360, 96, seed=43), None, 1)]
+        for case, f, refresh in specs:
+            ctx = capi.Context(case, forcing=f, shock_capturing=1, C_mu=c[0], C_beta=c[1], C_kappa=c[2])
+            if refresh:
+                ctx.set_learned_lad(model, 0.05, 1e-8, refresh)
+            out.append(ctx)
+        return out
+
+    alone = make()
+    for x in alone:
+        x.run(7)
+    batched = make()
+    b = capi.Batch(batched)
+    print("LAD batch of", len(batched), "runs, 7 steps", flush=True)
+    b.run(7)
+    for k, (x, y) in enumerate(zip(alone, batched)):
+        assert np.array_equal(x.dt_log()[0], y.dt_log()[0]), k
+        assert np.array_equal(x.state(), y.state()), k
+    b.close()
 
 
 def test_batch_needs_equal_step_parity():
