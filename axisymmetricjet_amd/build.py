@@ -4,6 +4,7 @@
 """
 from __future__ import annotations
 
+import json
 import os
 import shlex
 import subprocess
@@ -23,13 +24,32 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
          "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every source (in parallel) and link libajet.so; raises on any compiler error."""
+TUNE = os.path.join(HERE, "tune.json")
+
+
+def tile_defines(tile=None):
+    """-D flags of the stage/LAD tile (z points, r points, threads of the stage kernel): the
+    given tile, else the one the autotuner recorded in tune.json, else the sources' 32 x 16 x 256."""
+    if tile is None and os.path.exists(TUNE):
+        with open(TUNE) as f:
+            tile = json.load(f).get("tile")
+    if not tile:
+        return []
+    tz, tr, th = (int(x) for x in tile)
+    return [f"-DAJET_TZ={tz}", f"-DAJET_TR={tr}", f"-DAJET_THREADS={th}"]
+
+
+def build(verbose: bool = False, tile=None, out: str | None = None) -> str:
+    """Compile every source (in parallel) and link libajet.so (or `out`); raises on any compiler
+    error.  tile = (z points, r points, threads) overrides the tuned tile."""
+    out = out or LIB
+    objdir = os.path.join(os.path.dirname(out), "_build") if out != LIB else os.path.join(HERE, "_build")
     objs, procs = [], []
-    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    os.makedirs(objdir, exist_ok=True)
+    defs = tile_defines(tile)
     for src in SOURCES:
-        obj = os.path.join(HERE, "_build", src + ".o")
-        cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+        obj = os.path.join(objdir, src + ".o")
+        cmd = [HIPCC] + FLAGS + defs + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".cpp"):
             cmd[1:1] = ["-x", "hip"]
         if verbose:
@@ -39,11 +59,11 @@ def build(verbose: bool = False) -> str:
     failed = [src for src, p in procs if p.wait() != 0]
     if failed:
         raise subprocess.CalledProcessError(1, "hipcc " + " ".join(failed))
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -83,3 +83,18 @@ def test_fortran_plot3d_roundtrip_is_byte_identical(tmp_path):
     q, tau = jc.read_soln(sout)
     assert tau == (12.0, 0.1, 0.2, 3.5)
     assert np.array_equal(q, case.q0)
+
+
+def test_tuned_tile_is_a_candidate_and_reaches_the_build():
+    import json
+    from axisymmetricjet_amd import autotune, build
+    with open(autotune.TUNE) as f:
+        doc = json.load(f)
+    assert tuple(doc["tile"]) in autotune.CANDIDATES
+    assert doc["identical_results"] is True
+    tz, tr, th = doc["tile"]
+    assert build.tile_defines() == [f"-DAJET_TZ={tz}", f"-DAJET_TR={tr}", f"-DAJET_THREADS={th}"]
+    assert build.tile_defines((64, 8, 256)) == ["-DAJET_TZ=64", "-DAJET_TR=8", "-DAJET_THREADS=256"]
+    # every timed candidate is listed, and the choice is the fastest on the stated metric
+    t = doc["timings_ms_per_step"]
+    assert min(t, key=lambda k: t[k]["lad"]) == "x".join(str(x) for x in doc["tile"])
