@@ -429,7 +429,8 @@ static MlpInfer mlp_args(ajet_ctx* c, const double* q) {
 // Learned mode: the network's coefficients at the first stage of every refresh-th step; the
 // dilatation sensor at the first stage of every step; at every stage the reference's LAD on the
 // filter blocks it flagged.
-static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStream_t s) {
+static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStream_t s, hipStream_t aux,
+                                 hipEvent_t fork, hipEvent_t join) {
   hipError_t e;
   if (stage == 1 && c->learned_steps % c->refresh == 0 &&
       (e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, s)) != hipSuccess)
@@ -444,18 +445,23 @@ static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStr
   l.zr_flags = c->zr_flags;
   l.src_mask = c->src_mask;
   l.str_mask = c->str_mask;
-  return launch_lad(l, s);
+  return launch_lad(l, s, aux, fork, join);
 }
 
 // The artificial coefficients of one stage of a run in LAD mode (the reference's or learned).
-static hipError_t launch_coefficients(ajet_ctx* c, int stage, const double* q, hipStream_t s) {
-  return c->learned ? launch_learned(c, stage, q, s) : launch_lad(lad_args(c, q), s);
+static hipError_t launch_coefficients(ajet_ctx* c, int stage, const double* q, hipStream_t s, hipStream_t aux,
+                                      hipEvent_t fork, hipEvent_t join) {
+  return c->learned ? launch_learned(c, stage, q, s, aux, fork, join) : launch_lad(lad_args(c, q), s, aux, fork, join);
 }
 
 static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
   const int mode = physics_mode(c->p);
   hipError_t e;
-  if (mode == MODE_LAD && (e = launch_coefficients(c, stage, a.qin, c->stream)) != hipSuccess) return e;
+  // the LAD launches stay on one stream: a join across streams costs ~10 us of idle GPU between
+  // dependent kernels, more than the boundary tiles' strain and source launches take
+  if (mode == MODE_LAD && (e = launch_coefficients(c, stage, a.qin, c->stream, nullptr, nullptr, nullptr)) !=
+                              hipSuccess)
+    return e;
   if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
   a.tiles = c->bnd_tiles;
@@ -863,7 +869,7 @@ static hipError_t batch_stage(ajet_batch* b, int stage, int dt_from_dmax, int pa
   if (b->mode == MODE_LAD) {
     for (ajet_ctx* c : b->runs) {
       const double* q = stage == 1 ? c->Q0 : (stage == 3 ? c->Q2 : c->Q1);
-      if ((e = launch_coefficients(c, stage, q, b->stream)) != hipSuccess) return e;
+      if ((e = launch_coefficients(c, stage, q, b->stream, nullptr, nullptr, nullptr)) != hipSuccess) return e;
     }
   }
   if ((e = hipEventRecord(b->ev_fork, b->stream)) != hipSuccess) return e;

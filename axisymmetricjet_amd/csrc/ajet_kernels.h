@@ -95,7 +95,10 @@ struct LadArgs {
 
 hipError_t upload_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
 hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
-hipError_t launch_lad(const LadArgs& a, hipStream_t s);
+// The four LAD launches on stream s; with aux (and two events) the boundary-tile strain and
+// source launches run on aux beside the interior ones.
+hipError_t launch_lad(const LadArgs& a, hipStream_t s, hipStream_t aux = nullptr, hipEvent_t fork = nullptr,
+                      hipEvent_t join = nullptr);
 int lad_zfilter_segment(int nz);
 // the dilatation sensor of the learned mode: flags[group(j) * nseg + i / seg] |= 1 where
 // max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min (flags cleared first); then the

@@ -94,13 +94,18 @@ constexpr int LH = 7;
 constexpr int LWZ = TZ + 2 * LH;
 constexpr int LWR = TR + 2 * LH;
 
+// The three fields pass through one LDS window in turn (|S|, dil, then T): 22 KB per block
+// instead of 62 KB for all three at once, so about seven blocks share a CU instead of two; each
+// point keeps its three pairs of fourth derivatives in registers across the passes.
 template <bool INT>
 __global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* __restrict__ tiles, int ntiles) {
-  __shared__ double sB[3][LWR][LWZ];   // |S|, dil, T
-  __shared__ double sZ[3][TR][TZ + 4];  // D2 along xi, columns i0-2 .. i0+TZ+1
-  __shared__ double sR[3][TR + 4][TZ];  // D2 along eta, rows j0-2 .. j0+TR+1
-  __shared__ uint8_t cZ[TR][TZ + 4];    // z row codes where the D2 along xi is taken
-  __shared__ uint8_t cR[TR + 4][TZ];    // r row codes where the D2 along eta is taken
+  __shared__ double sB[LWR][LWZ];     // the field of this pass
+  __shared__ double sZ[TR][TZ + 4];   // its D2 along xi, columns i0-2 .. i0+TZ+1
+  __shared__ double sR[TR + 4][TZ];   // its D2 along eta, rows j0-2 .. j0+TR+1
+  __shared__ uint8_t cZ[TR][TZ + 4];  // z row codes where the D2 along xi is taken
+  __shared__ uint8_t cR[TR + 4][TZ];  // r row codes where the D2 along eta is taken
+  constexpr int ROWS = 256 / TZ;                 // rows of owned points per pass of the block
+  constexpr int NP = (TR + ROWS - 1) / ROWS;     // owned points per thread
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
@@ -110,16 +115,6 @@ __global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* _
   if (a.src_mask && !a.src_mask[t]) return;
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
-  for (int k = threadIdx.x; k < LWZ * LWR; k += blockDim.x) {
-    const int wz = k % LWZ, wr = k / LWZ;
-    const int i = i0 - LH + wz, j = j0 - LH + wr;
-    const bool in = i >= 0 && i < nz && j >= 0 && j < nr;
-    const long long gi = in ? (long long)i + (long long)pitch * j : 0;
-    const double s = a.A[gi], d = a.A[gi + fs], T = a.q[gi + 4 * fs];
-    sB[0][wr][wz] = in ? s : 0.0;
-    sB[1][wr][wz] = in ? d : 0.0;
-    sB[2][wr][wz] = in ? T : 0.0;
-  }
   if (!INT) {
     // codes of the points where a second derivative is taken; outside the grid it is zero,
     // which the blank code gives
@@ -138,40 +133,55 @@ __global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* _
       cR[w][c] = (uint8_t)(in ? ((m >> 4) & 15u) : ROW_BLANK);
     }
   }
-  __syncthreads();
-  for (int k = threadIdx.x; k < TR * (TZ + 4); k += blockDim.x) {
-    const int c = k % (TZ + 4), ty = k / (TZ + 4);
-    const uint32_t zr = INT ? ROW_INTERIOR : cZ[ty][c];
-#pragma unroll
-    for (int f = 0; f < 3; ++f) sZ[f][ty][c] = d2_sum(zr, &sB[f][ty + LH][c - 2 + LH], 1);
-  }
-  for (int k = threadIdx.x; k < (TR + 4) * TZ; k += blockDim.x) {
-    const int c = k % TZ, w = k / TZ;
-    const uint32_t rr = INT ? ROW_INTERIOR : cR[w][c];
-#pragma unroll
-    for (int f = 0; f < 3; ++f) sR[f][w][c] = d2_sum(rr, &sB[f][w - 2 + LH][c + LH], LWZ);
-  }
-  __syncthreads();
   const int tz = threadIdx.x % TZ;
-  for (int ty = threadIdx.x / TZ; ty < TR; ty += blockDim.x / TZ) {
-    const int i = i0 + tz, j = j0 + ty;
-    if (i >= i1 || j >= j1) continue;
-    const long long gi = (long long)i + (long long)pitch * j;
-    const uint32_t zr = INT ? ROW_INTERIOR : cZ[ty][tz + 2], rr = INT ? ROW_INTERIOR : cR[ty + 2][tz];
-    double dz4[3], dr4[3];
+  double dz4[3][NP], dr4[3][NP], Tc[NP];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      dz4[f] = d2_sum(zr, &sZ[f][ty][tz + 2], 1);
-      dr4[f] = d2_sum(rr, &sR[f][ty + 2][tz], TZ);
+  for (int f = 0; f < 3; ++f) {
+    if (f > 0) __syncthreads();  // the previous field's stencils are read
+    const double* __restrict__ src = f == 2 ? a.q + 4 * fs : a.A + f * fs;
+    for (int k = threadIdx.x; k < LWZ * LWR; k += blockDim.x) {
+      const int wz = k % LWZ, wr = k / LWZ;
+      const int i = i0 - LH + wz, j = j0 - LH + wr;
+      const bool in = i >= 0 && i < nz && j >= 0 && j < nr;
+      const double v = src[in ? (long long)i + (long long)pitch * j : 0];
+      sB[wr][wz] = in ? v : 0.0;
     }
+    __syncthreads();
+    for (int k = threadIdx.x; k < TR * (TZ + 4); k += blockDim.x) {
+      const int c = k % (TZ + 4), ty = k / (TZ + 4);
+      const uint32_t zr = INT ? ROW_INTERIOR : cZ[ty][c];
+      sZ[ty][c] = d2_sum(zr, &sB[ty + LH][c - 2 + LH], 1);
+    }
+    for (int k = threadIdx.x; k < (TR + 4) * TZ; k += blockDim.x) {
+      const int c = k % TZ, w = k / TZ;
+      const uint32_t rr = INT ? ROW_INTERIOR : cR[w][c];
+      sR[w][c] = d2_sum(rr, &sB[w - 2 + LH][c + LH], LWZ);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int ty = (int)threadIdx.x / TZ + p * ROWS;
+      if (ty >= TR) continue;
+      const uint32_t zr = INT ? ROW_INTERIOR : cZ[ty][tz + 2], rr = INT ? ROW_INTERIOR : cR[ty + 2][tz];
+      dz4[f][p] = d2_sum(zr, &sZ[ty][tz + 2], 1);
+      dr4[f][p] = d2_sum(rr, &sR[ty + 2][tz], TZ);
+      if (f == 2) Tc[p] = sB[ty + LH][tz + LH];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int ty = (int)threadIdx.x / TZ + p * ROWS;
+    const int i = i0 + tz, j = j0 + ty;
+    if (ty >= TR || i >= i1 || j >= j1) continue;
+    const long long gi = (long long)i + (long long)pitch * j;
     const double rho = a.q[gi];
     const double fsw = a.A[gi + 2 * fs];
-    const double T = sB[2][ty + LH][tz + LH];
+    const double T = Tc[p];
     const double dz = a.dzl[i], dr = a.drl[j];
     const double dz2 = dz * dz, dr2 = dr * dr;
-    a.B[gi] = rho * fabs(dz4[0] * dz2 + dr4[0] * dr2);
-    a.B[gi + fs] = rho * fsw * fabs(dz4[1] * dz2 + dr4[1] * dr2);
-    a.B[gi + 2 * fs] = rho / (a.gam * sqrt(T)) * fabs(dz4[2] * dz + dr4[2] * dr);
+    a.B[gi] = rho * fabs(dz4[0][p] * dz2 + dr4[0][p] * dr2);
+    a.B[gi + fs] = rho * fsw * fabs(dz4[1][p] * dz2 + dr4[1][p] * dr2);
+    a.B[gi + 2 * fs] = rho / (a.gam * sqrt(T)) * fabs(dz4[2][p] * dz + dr4[2][p] * dr);
   }
 }
 
@@ -507,13 +517,26 @@ hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], c
   return e;
 }
 
-hipError_t launch_lad(const LadArgs& a, hipStream_t s) {
+hipError_t launch_lad(const LadArgs& a, hipStream_t s, hipStream_t aux, hipEvent_t fork, hipEvent_t join) {
   const Geometry& g = a.g;
   auto grid = [](int n) { return dim3((unsigned)((n + 7) / 8 * 8)); };
-  if (a.n_int > 0) hipLaunchKernelGGL(lad_strain_kernel<true>, grid(a.n_int), dim3(256), 0, s, a, a.int_tiles, a.n_int);
-  if (a.n_bnd > 0) hipLaunchKernelGGL(lad_strain_kernel<false>, grid(a.n_bnd), dim3(256), 0, s, a, a.bnd_tiles, a.n_bnd);
-  if (a.n_int > 0) hipLaunchKernelGGL(lad_source_kernel<true>, grid(a.n_int), dim3(256), 0, s, a, a.int_tiles, a.n_int);
-  if (a.n_bnd > 0) hipLaunchKernelGGL(lad_source_kernel<false>, grid(a.n_bnd), dim3(256), 0, s, a, a.bnd_tiles, a.n_bnd);
+  hipError_t e;
+  // boundary tiles on the second stream beside the interior tiles (when one is given); the
+  // sources read the strain rate of neighbouring tiles, so each kernel pair joins before the next
+  auto pair = [&](auto kint, auto kbnd) -> hipError_t {
+    hipStream_t sb = aux ? aux : s;
+    if (aux && a.n_bnd > 0) {
+      if ((e = hipEventRecord(fork, s)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(aux, fork, 0)) != hipSuccess) return e;
+    }
+    if (a.n_bnd > 0) hipLaunchKernelGGL(kbnd, grid(a.n_bnd), dim3(256), 0, sb, a, a.bnd_tiles, a.n_bnd);
+    if (aux && a.n_bnd > 0 && (e = hipEventRecord(join, aux)) != hipSuccess) return e;
+    if (a.n_int > 0) hipLaunchKernelGGL(kint, grid(a.n_int), dim3(256), 0, s, a, a.int_tiles, a.n_int);
+    if (aux && a.n_bnd > 0 && (e = hipStreamWaitEvent(s, join, 0)) != hipSuccess) return e;
+    return hipGetLastError();
+  };
+  if ((e = pair(lad_strain_kernel<true>, lad_strain_kernel<false>)) != hipSuccess) return e;
+  if ((e = pair(lad_source_kernel<true>, lad_source_kernel<false>)) != hipSuccess) return e;
   const int seg = zfilter_segment(g.nz);
   hipLaunchKernelGGL(lad_zrfilter_kernel, dim3((g.nz + seg - 1) / seg, a.nrgroups, 3), dim3(ZLINES), 0, s, a, seg);
   return hipGetLastError();
