@@ -55,6 +55,12 @@ struct ajet_ctx {
   double sensor_thr = 0.0, ramp_min = 0.0;
   int refresh = 1;                         // network every refresh steps
   long long learned_steps = 0;             // steps since the network last ran (0: run it next)
+  uint32_t gen = 0;                        // stamp of the sensor's flags (one per step)
+  bool sensor_ready = false;               // the sensor of the current state ran after the last step
+  bool skip_lad = false;                   // this step flagged no block: no LAD launches
+  int* count_dev = nullptr;                // flagged blocks, on the device
+  int* count_host = nullptr;               // ... and in pinned host memory
+  hipEvent_t ev_sensor = nullptr;          // the count has arrived
   std::vector<int> lad_groups_h;           // host copy of the line groups
   int nseg = 0;                            // z segments of the filter blocks
   uint32_t* zr_flags = nullptr;            // flagged filter blocks (segment + nseg * group)
@@ -136,10 +142,12 @@ int ajet_destroy(ajet_ctx* c) {
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
                   c->int_tiles, c->bnd_tiles, c->lad_int, c->lad_bnd, c->lad_groups, c->codes, c->sc, c->dtlog,
                   c->mlpW, c->mlpB, c->mlpShift, c->mlpScale, c->zr_flags, c->src_mask, c->str_mask,
-                  c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out};
+                  c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out, c->count_dev};
   if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  if (c->count_host) (void)hipHostFree(c->count_host);
+  if (c->ev_sensor) (void)hipEventDestroy(c->ev_sensor);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->aux) (void)hipStreamDestroy(c->aux);
@@ -296,6 +304,7 @@ int ajet_set_state(ajet_ctx* c, const double* pv, const double* pv_target) {
   copy_state_in(c, c->Q0, pv);
   if (pv_target) copy_state_in(c, c->tgt, pv_target);
   c->learned_steps = 0;
+  c->sensor_ready = false;
   StepScalars z{};
   HIPCHK(c, hipMemcpyAsync(c->sc, &z, sizeof z, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -429,6 +438,28 @@ static MlpInfer mlp_args(ajet_ctx* c, const double* q) {
 // Learned mode: the network's coefficients at the first stage of every refresh-th step; the
 // dilatation sensor at the first stage of every step; at every stage the reference's LAD on the
 // filter blocks it flagged.
+// The dilatation sensor, the tile masks and the count of flagged blocks for the state q, under
+// a new stamp; the count goes to pinned host memory behind ev_sensor.
+static hipError_t learned_sensor(ajet_ctx* c, const double* q, hipStream_t s) {
+  hipError_t e;
+  LadArgs l = lad_args(c, q);
+  l.nseg = c->nseg;
+  l.gen = ++c->gen;
+  if ((e = launch_lad_sensor(l, c->zr_flags, c->line_group, c->sensor_thr, c->ramp_min, s)) != hipSuccess) return e;
+  if ((e = launch_lad_masks(l, c->zr_flags, c->src_mask, c->str_mask, s)) != hipSuccess) return e;
+  if ((e = launch_lad_count(c->zr_flags, c->nseg * c->n_lad_groups, c->gen, c->count_dev, s)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(c->count_host, c->count_dev, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  return hipEventRecord(c->ev_sensor, s);
+}
+
+// After the last stage of a step: the sensor of the new state, so that the next step knows
+// before its first launch whether any block falls back to the reference's LAD.
+static hipError_t learned_after_step(ajet_ctx* c, hipStream_t s) {
+  hipError_t e = learned_sensor(c, c->Q0, s);
+  c->sensor_ready = e == hipSuccess;
+  return e;
+}
+
 static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStream_t s, hipStream_t aux,
                                  hipEvent_t fork, hipEvent_t join) {
   hipError_t e;
@@ -436,12 +467,17 @@ static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStr
       (e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, s)) != hipSuccess)
     return e;
   if (stage == 4) ++c->learned_steps;
+  if (stage == 1) {
+    // the sensor of this state: taken after the previous step (wait for its count), or now
+    if (!c->sensor_ready && (e = learned_sensor(c, q, s)) != hipSuccess) return e;
+    c->sensor_ready = false;
+    if ((e = hipEventSynchronize(c->ev_sensor)) != hipSuccess) return e;
+    c->skip_lad = *c->count_host == 0;
+  }
+  if (c->skip_lad) return hipSuccess;  // no block flagged: the LAD launches would all return at once
   LadArgs l = lad_args(c, q);
   l.nseg = c->nseg;
-  if (stage == 1) {
-    if ((e = launch_lad_sensor(l, c->zr_flags, c->line_group, c->sensor_thr, c->ramp_min, s)) != hipSuccess) return e;
-    if ((e = launch_lad_masks(l, c->zr_flags, c->src_mask, c->str_mask, s)) != hipSuccess) return e;
-  }
+  l.gen = c->gen;
   l.zr_flags = c->zr_flags;
   l.src_mask = c->src_mask;
   l.str_mask = c->str_mask;
@@ -471,7 +507,9 @@ static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
   a.tiles = c->int_tiles;
   a.ntiles = c->n_int;
   if ((e = launch_stage(stage, mode, true, a, c->stream)) != hipSuccess) return e;
-  return hipStreamWaitEvent(c->stream, c->ev_join, 0);
+  if ((e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;
+  if (mode == MODE_LAD && c->learned && stage == 4) return learned_after_step(c, c->stream);
+  return hipSuccess;
 }
 
 static int read_dt(ajet_ctx* c, double* dt) {
@@ -600,6 +638,7 @@ int ajet_lad_coeffs(ajet_ctx* c, const double* q, double* mustar, double* betast
   HIPCHK(c, hipSetDevice(c->device));
   copy_state_in(c, c->Q2, q);
   c->learned_steps = 0;  // the coefficient buffer is overwritten: the next step recomputes
+  c->sensor_ready = false;
   HIPCHK(c, launch_lad(lad_args(c, c->Q2), c->stream));
   double* outs[3] = {mustar, betastar, kappastar};
   for (int f = 0; f < 3; ++f)
@@ -645,6 +684,7 @@ int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_thres
                          int32_t refresh) {
   if (!c || !c->stream) return AJET_EINVAL;
   c->learned_steps = 0;
+  c->sensor_ready = false;
   if (!model) {
     c->learned = false;
     return AJET_OK;
@@ -662,6 +702,11 @@ int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_thres
     HIPCHK(c, hipMalloc((void**)&c->mlpScale, sizeof(float) * MLP_NIN));
     c->nseg = (g.nz + lad_zfilter_segment(g.nz) - 1) / lad_zfilter_segment(g.nz);
     HIPCHK(c, hipMalloc((void**)&c->zr_flags, sizeof(uint32_t) * (size_t)c->nseg * c->n_lad_groups));
+    HIPCHK(c, hipMemset(c->zr_flags, 0, sizeof(uint32_t) * (size_t)c->nseg * c->n_lad_groups));
+    HIPCHK(c, hipMalloc((void**)&c->count_dev, sizeof(int)));
+    HIPCHK(c, hipHostMalloc((void**)&c->count_host, sizeof(int), hipHostMallocDefault));
+    *c->count_host = 0;
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_sensor, hipEventDisableTiming));
     HIPCHK(c, hipMalloc((void**)&c->src_mask, (size_t)g.ntz * g.ntr));
     HIPCHK(c, hipMalloc((void**)&c->str_mask, (size_t)g.ntz * g.ntr));
     std::vector<int> lg((size_t)g.nr, 0);
@@ -719,6 +764,7 @@ int ajet_lad_learned_coeffs(ajet_ctx* c, const double* q, double* mustar, double
   HIPCHK(c, hipSetDevice(c->device));
   copy_state_in(c, c->Q2, q);
   c->learned_steps = 0;
+  c->sensor_ready = false;
   HIPCHK(c, launch_mlp_infer(mlp_args(c, c->Q2), c->infer_blocks, c->stream));
   double* outs[3] = {mustar, betastar, kappastar};
   for (int f = 0; f < 3; ++f)
@@ -736,7 +782,7 @@ int ajet_lad_fallback_fraction(ajet_ctx* c, double* fraction) {
   HIPCHK(c, hipMemcpyAsync(f.data(), c->zr_flags, sizeof(uint32_t) * f.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   size_t k = 0;
-  for (uint32_t v : f) k += v != 0u;
+  for (uint32_t v : f) k += v == c->gen && c->gen != 0u;
   *fraction = f.empty() ? 0.0 : (double)k / (double)f.size();
   return AJET_OK;
 }
@@ -881,7 +927,11 @@ static hipError_t batch_stage(ajet_batch* b, int stage, int dt_from_dmax, int pa
   if ((e = launch_stage_batch(stage, b->mode, true, b->force, b->args[stage - 1], b->d_int, b->n_int, dt_from_dmax,
                               parity, b->stream)) != hipSuccess)
     return e;
-  return hipStreamWaitEvent(b->stream, b->ev_join, 0);
+  if ((e = hipStreamWaitEvent(b->stream, b->ev_join, 0)) != hipSuccess) return e;
+  if (b->mode == MODE_LAD && stage == 4)
+    for (ajet_ctx* c : b->runs)
+      if (c->learned && (e = learned_after_step(c, b->stream)) != hipSuccess) return e;
+  return hipSuccess;
 }
 
 int ajet_batch_run(ajet_batch* b, int32_t nsteps) {

@@ -91,6 +91,7 @@ struct LadArgs {
   const uint8_t* src_mask;  // per tile: the sources are needed
   const uint8_t* str_mask;  // per tile: the strain rate and dilatation are needed
   int nseg;
+  uint32_t gen;             // the step's stamp: a block is flagged when its flag equals it
 };
 
 hipError_t upload_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
@@ -100,13 +101,14 @@ hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], c
 hipError_t launch_lad(const LadArgs& a, hipStream_t s, hipStream_t aux = nullptr, hipEvent_t fork = nullptr,
                       hipEvent_t join = nullptr);
 int lad_zfilter_segment(int nz);
-// the dilatation sensor of the learned mode: flags[group(j) * nseg + i / seg] |= 1 where
-// max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min (flags cleared first); then the
-// tile masks of the flagged blocks
+// the dilatation sensor of the learned mode: flags[group(j) * nseg + i / seg] = gen where
+// max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min; then the tile masks of the
+// blocks so flagged
 hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, const int* line_group, double thr, double ramp_min,
                              hipStream_t s);
 hipError_t launch_lad_masks(const LadArgs& a, const uint32_t* flags, uint8_t* src_mask, uint8_t* str_mask,
                             hipStream_t s);
+hipError_t launch_lad_count(const uint32_t* flags, int n, uint32_t gen, int* count, hipStream_t s);
 
 // ---- learned LAD coefficient (ajet_mlp.hip): 104 features -> 128 -> 128 -> 128 -> 2.
 constexpr int MLP_NIN = 104;       // features

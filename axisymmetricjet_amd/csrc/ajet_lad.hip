@@ -27,10 +27,10 @@ __global__ __launch_bounds__(256) void lad_strain_kernel(LadArgs a, const int* _
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
-  const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  if (li >= ntiles) return;
+  // one tile per block, or (a grid smaller than the list) several in turn
+  for (int li = xcd_tile((int)blockIdx.x, (int)gridDim.x); li < ntiles; li += (int)gridDim.x) {
   const int t = tiles[li];
-  if (a.str_mask && !a.str_mask[t]) return;
+  if (a.str_mask && !a.str_mask[t]) continue;
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
   for (int k = threadIdx.x; k < WZ * WR; k += blockDim.x) {
@@ -83,6 +83,8 @@ __global__ __launch_bounds__(256) void lad_strain_kernel(LadArgs a, const int* _
     a.A[gi + fs] = dil;
     a.A[gi + 2 * fs] = fsw;
   }
+  __syncthreads();  // the window is reused by the next tile
+  }
 }
 
 // ---- D2(D2(.)) along xi and eta of |S|, dil and T, and the three quantities
@@ -109,10 +111,10 @@ __global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* _
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
-  const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
-  if (li >= ntiles) return;
+  // one tile per block, or (a grid smaller than the list) several in turn
+  for (int li = xcd_tile((int)blockIdx.x, (int)gridDim.x); li < ntiles; li += (int)gridDim.x) {
   const int t = tiles[li];
-  if (a.src_mask && !a.src_mask[t]) return;
+  if (a.src_mask && !a.src_mask[t]) continue;
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
   if (!INT) {
@@ -182,6 +184,8 @@ __global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* _
     a.B[gi] = rho * fabs(dz4[0][p] * dz2 + dr4[0][p] * dr2);
     a.B[gi + fs] = rho * fsw * fabs(dz4[1][p] * dz2 + dr4[1][p] * dr2);
     a.B[gi + 2 * fs] = rho / (a.gam * sqrt(T)) * fabs(dz4[2][p] * dz + dr4[2][p] * dr);
+  }
+  __syncthreads();  // the window and codes are reused by the next tile
   }
 }
 
@@ -265,7 +269,7 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
   const int S1 = min(nz, S0 + seg);
   const int jb = a.rgroups[3 * blockIdx.y];
   const int out_lo = a.rgroups[3 * blockIdx.y + 1], out_hi = a.rgroups[3 * blockIdx.y + 2];
-  if (a.zr_flags && !a.zr_flags[blockIdx.x + (unsigned)a.nseg * blockIdx.y]) return;
+  if (a.zr_flags && a.zr_flags[blockIdx.x + (unsigned)a.nseg * blockIdx.y] != a.gen) return;
   const int f = (int)blockIdx.z;
   const double* __restrict__ x = a.B + f * fs;
   double* __restrict__ out = a.out + f * fs;
@@ -433,11 +437,6 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
 }
 
 // ---- learned mode: the dilatation sensor and the tile masks of the flagged filter blocks.
-__global__ __launch_bounds__(256) void lad_clear_flags_kernel(uint32_t* flags, int n) {
-  const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  if (k < n) flags[k] = 0u;
-}
-
 __global__ __launch_bounds__(256) void lad_sensor_kernel(LadArgs a, uint32_t* __restrict__ flags,
                                                          const int* __restrict__ line_group, int seg, double thr,
                                                          double ramp_min) {
@@ -456,8 +455,9 @@ __global__ __launch_bounds__(256) void lad_sensor_kernel(LadArgs a, uint32_t* __
   const double dil = dVzdz + dVrdr + (r > 0.0 ? at(1, i, j) / r : dVrdr);  // 2 dVr/dr on the axis
   const double s = fmax(0.0, -dil) * fmax(a.dzl[i], a.drl[j]) / sqrt(at(4, i, j));
   if (s > thr) {
+    // the step's stamp: every writer stores the same value, and older stamps need no clearing
     uint32_t* f = &flags[line_group[j] * a.nseg + i / seg];
-    if (*f == 0u) atomicOr(f, 1u);
+    if (*f != a.gen) *f = a.gen;
   }
 }
 
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(256) void lad_mask_kernel(LadArgs a, const uint32_t
     const bool rsrc = j0 < wb + ZLINES && j1 > wb, rstr = j0 < wb + ZLINES + DR && j1 > wb - DR;
     if (!rstr) continue;
     for (int sg = sg_lo; sg <= sg_hi; ++sg) {
-      if (!flags[sg + a.nseg * gr]) continue;
+      if (flags[sg + a.nseg * gr] != a.gen) continue;
       const int S0 = sg * seg, S1 = min(g.nz, S0 + seg), ib = max(0, S0 - ZWARM);
       const int zlo = ib - 16, zhi = S1 + 32;
       src = src || (rsrc && i0 < zhi && i1 > zlo);
@@ -490,12 +490,30 @@ __global__ __launch_bounds__(256) void lad_mask_kernel(LadArgs a, const uint32_t
   str_mask[t] = str ? 1 : 0;
 }
 
+// the number of flagged blocks (flags equal to gen), one block of 256 threads
+__global__ __launch_bounds__(256) void lad_count_kernel(const uint32_t* __restrict__ flags, int n, uint32_t gen,
+                                                        int* __restrict__ count) {
+  __shared__ int part[256];
+  int c = 0;
+  for (int k = (int)threadIdx.x; k < n; k += 256) c += flags[k] == gen ? 1 : 0;
+  part[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *count = part[0];
+}
+
+hipError_t launch_lad_count(const uint32_t* flags, int n, uint32_t gen, int* count, hipStream_t s) {
+  hipLaunchKernelGGL(lad_count_kernel, dim3(1), dim3(256), 0, s, flags, n, gen, count);
+  return hipGetLastError();
+}
+
 int lad_zfilter_segment(int nz) { return zfilter_segment(nz); }
 
 hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, const int* line_group, double thr, double ramp_min,
                              hipStream_t s) {
-  const int nflags = a.nseg * a.nrgroups;
-  hipLaunchKernelGGL(lad_clear_flags_kernel, dim3((unsigned)((nflags + 255) / 256)), dim3(256), 0, s, flags, nflags);
   const long long n = (long long)a.g.nz * a.g.nr;
   hipLaunchKernelGGL(lad_sensor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, flags, line_group,
                      zfilter_segment(a.g.nz), thr, ramp_min);
@@ -519,7 +537,11 @@ hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], c
 
 hipError_t launch_lad(const LadArgs& a, hipStream_t s, hipStream_t aux, hipEvent_t fork, hipEvent_t join) {
   const Geometry& g = a.g;
-  auto grid = [](int n) { return dim3((unsigned)((n + 7) / 8 * 8)); };
+  // masked (learned mode): most tiles return at once, so a grid of at most MASKED_GRID blocks
+  // walks the list instead of one block per tile
+  constexpr int MASKED_GRID = 1024;
+  const bool masked = a.src_mask != nullptr;
+  auto grid = [&](int n) { return dim3((unsigned)(((masked && n > MASKED_GRID ? MASKED_GRID : n) + 7) / 8 * 8)); };
   hipError_t e;
   // boundary tiles on the second stream beside the interior tiles (when one is given); the
   // sources read the strain rate of neighbouring tiles, so each kernel pair joins before the next

@@ -200,15 +200,18 @@ typedef struct ajet_mlp {
  * dilatation sensor s = max(0, -dil) max(dz, dr) / sqrt(T) is taken, and every z/r filter block
  * (a z segment times an r line group) holding a point with s > sensor_threshold and
  * ramp > ramp_min gets the reference's coefficients at each of the step's four stages instead:
- * the LAD kernels run only for those blocks and the tiles that feed them.  sensor_threshold < 0
- * reproduces the reference's LAD exactly. */
+ * the LAD kernels run only for those blocks and the tiles that feed them, and not at all in a
+ * step that flags none.  The sensor of a state is taken after the step that produced it (same
+ * state, same flags), so the host knows the count before the next step's first launch: each
+ * step then waits once for the previous one.  sensor_threshold < 0 reproduces the reference's
+ * LAD exactly. */
 int ajet_set_learned_lad(ajet_ctx* ctx, const ajet_mlp* model, double sensor_threshold, double ramp_min,
                          int32_t refresh);
 
 /* Test and training hooks: the raw features (n x 104, row major) of the points (i, j), 0-based
  * (points holds 2 n int32), of the state q (nz, nr, 5); the network's mu*, beta*, kappa* of
- * the state q (no fallback), each (nz, nr); the fraction of filter blocks the sensor sent to
- * the reference's LAD in the last step taken. */
+ * the state q (no fallback), each (nz, nr); the fraction of filter blocks the sensor flags
+ * for the current state (taken after each step, for the next one). */
 int ajet_lad_features(ajet_ctx* ctx, const double* q, const int32_t* points, int32_t n, float* x);
 int ajet_lad_learned_coeffs(ajet_ctx* ctx, const double* q, double* mustar, double* betastar,
                             double* kappastar);
