@@ -236,7 +236,7 @@ __global__ __launch_bounds__(256) void mlp_features_kernel(MlpInfer a, const int
 // outputs 16 t + 4 q .. + 3 of point p, which the epilogue (bias, GELU, BF16) writes back over
 // the layer's input as one 8-byte store.
 constexpr int WROW = MLP_K + 8;  // padded LDS row (BF16 elements): 16-byte reads conflict-free
-constexpr int INFER_WAVES = 8;
+constexpr int INFER_WAVES = 12;  // 3 waves per SIMD: 160 KB of LDS holds the weights and 12 activation tiles
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void wave_order() {
@@ -248,14 +248,14 @@ __device__ __forceinline__ void wave_order() {
 }
 
 __global__ __launch_bounds__(64 * INFER_WAVES) void mlp_infer_kernel(MlpInfer a) {
-  __shared__ __bf16 sW[3 * MLP_H + 16][WROW];            // W1 (K padded), W2, W3, W4 (rows >= 2 zero)
+  __shared__ __bf16 sW[3 * MLP_H + MLP_NOUT][WROW];      // W1 (K padded), W2, W3, the two rows of W4
   __shared__ __bf16 sAct[INFER_WAVES][16][WROW];
   __shared__ __attribute__((aligned(16))) float sBias[3 * MLP_H + MLP_NOUT];
   __shared__ float sShift[MLP_NIN], sScale[MLP_NIN];
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
-  for (int k = (int)threadIdx.x; k < (3 * MLP_H + 16) * (MLP_K / 8); k += blockDim.x) {
+  for (int k = (int)threadIdx.x; k < (3 * MLP_H + MLP_NOUT) * (MLP_K / 8); k += blockDim.x) {
     const int r = k / (MLP_K / 8), c = (k % (MLP_K / 8)) * 8;
     *reinterpret_cast<bf16x8*>(&sW[r][c]) = *reinterpret_cast<const bf16x8*>(a.W + (long long)r * MLP_K + c);
   }
@@ -334,10 +334,12 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void mlp_infer_kernel(MlpInfer a)
       wave_order();
     }
     f32x4 y = {0.0f, 0.0f, 0.0f, 0.0f};
+    const bf16x8 zero8 = {};
 #pragma unroll
     for (int kk = 0; kk < 4; ++kk) {
       const bf16x8 xf = *reinterpret_cast<const bf16x8*>(&act[p][32 * kk + 8 * q]);
-      const bf16x8 wf = *reinterpret_cast<const bf16x8*>(&sW[3 * MLP_H + p][32 * kk + 8 * q]);
+      // rows 2 .. 15 of the output tile are zero weights
+      const bf16x8 wf = p < MLP_NOUT ? *reinterpret_cast<const bf16x8*>(&sW[3 * MLP_H + p][32 * kk + 8 * q]) : zero8;
       y = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf, xf, y, 0, 0, 0);
     }
     // lanes q = 0 hold outputs 0 (beta*) and 1 (kappa*) of point p; mu* is zero
