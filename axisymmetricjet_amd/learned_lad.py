@@ -51,7 +51,7 @@ def snapshot_samples(ctx, q, zc, rng, n):
     return x, t
 
 
-def generate(machs=MACHS, nz=4400, nr=640, warm=200, every=200, nsnap=8, per_snap=4096, seed=7, attempts=4,
+def generate(machs=MACHS, nz=4400, nr=640, warm=200, every=150, nsnap=12, per_snap=8192, seed=7, attempts=4,
              log=print):
     """Samples from plain-LAD runs at each Mach number: nsnap snapshots every `every` steps
     after `warm` steps.  Some noise seeds make the reference's scheme blow up at the corner of
@@ -113,7 +113,7 @@ def init_model(seed=0):
     return m
 
 
-def train(x, t, steps=(3000, 1500, 1500), lrs=(2e-3, 5e-4, 1e-4), batch=4096, seed=0, log=print):
+def train(x, t, steps=(6000, 3000, 3000), lrs=(2e-3, 5e-4, 1e-4), batch=4096, seed=0, log=print):
     """Fit the network to raw features x and targets t on the GPU; returns (model, losses)."""
     m = init_model(seed)
     m["shift"], m["scale"], m["out_scale"] = normalisation(x, t)
@@ -238,11 +238,11 @@ def main(argv=None):
     ap.add_argument("--nz", type=int, default=4400)
     ap.add_argument("--nr", type=int, default=640)
     ap.add_argument("--warm", type=int, default=200)
-    ap.add_argument("--every", type=int, default=200)
-    ap.add_argument("--nsnap", type=int, default=8)
-    ap.add_argument("--per-snap", type=int, default=4096)
+    ap.add_argument("--every", type=int, default=150)
+    ap.add_argument("--nsnap", type=int, default=12)
+    ap.add_argument("--per-snap", type=int, default=8192)
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--steps", type=int, nargs=3, default=(3000, 1500, 1500))
+    ap.add_argument("--steps", type=int, nargs=3, default=(6000, 3000, 3000))
     ap.add_argument("--seed", type=int, default=7)
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731

@@ -1,0 +1,47 @@
+"""What the trained network gains over the method without it, on the 4400 x 640 Mach 1.5 jet
+(the benchmark's run, noise seed 1234, which is not among the training seeds):
+
+* its beta* and kappa* fields against the reference LAD's, beside an untrained network and zero
+  coefficients (the learned mode without the network keeps only the sensor's fallback blocks);
+* a scoring run: far-field OASPL on the arc of probes against the reference-LAD run, for the
+  learned mode and for the learned mode without the network."""
+import numpy as np
+import pytest
+
+from axisymmetricjet_amd import capi
+from axisymmetricjet_amd import jetcase as jc
+from axisymmetricjet_amd import learned_lad as ll
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def case():
+    return jc.make_jet(4400, 640, seed=1234)
+
+
+def test_trained_coefficients_beat_untrained_and_zero(case):
+    ctx = capi.Context(case, **ll.LAD)
+    ctx.run(400)
+    q = ctx.state()
+    trained = ll.coefficient_errors(ctx, q, ll.load())
+    untrained = ll.init_model(3)
+    untrained["shift"], untrained["scale"], untrained["out_scale"] = (ll.load()[k] for k in ("shift", "scale", "out_scale"))
+    raw = ll.coefficient_errors(ctx, q, untrained)
+    print(f"relative L2 error: trained beta* {trained['beta']:.3f} kappa* {trained['kappa']:.3f}; "
+          f"untrained {raw['beta']:.3f} {raw['kappa']:.3f}; zero coefficients 1", flush=True)
+    assert trained["beta"] < 0.6 and trained["kappa"] < 0.4
+    assert trained["beta"] < 0.5 * raw["beta"] and trained["kappa"] < 0.5 * raw["kappa"]
+
+
+def test_learned_mode_acoustics_beat_the_method_without_the_network(case):
+    res = ll.score(case, ll.load(), sensor_threshold=0.1, steps=1200, every=4, refresh=4)
+    assert res["reference_finite"]
+    lr, zr = res["learned"], res["zero"]
+    print(f"OASPL error vs reference LAD, 1200 steps: learned max {lr['oaspl_err_db_max']:.3f} dB "
+          f"mean {lr['oaspl_err_db_mean']:.3f}; without the network max {zr['oaspl_err_db_max']:.3f} "
+          f"mean {zr['oaspl_err_db_mean']:.3f}", flush=True)
+    assert lr["finite"] and zr["finite"]
+    assert lr["oaspl_err_db_max"] <= 0.5
+    assert lr["oaspl_err_db_mean"] < zr["oaspl_err_db_mean"]
+    assert lr["spectrum_err_db"] < zr["spectrum_err_db"]
