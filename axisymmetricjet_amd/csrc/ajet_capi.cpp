@@ -37,6 +37,7 @@ struct ajet_ctx {
   int n_lad_int = 0, n_lad_bnd = 0;
   int* lad_groups = nullptr;                       // line groups of the fused z/r filter
   int n_lad_groups = 0;
+  int lad_seg_first = 0, lad_src_imin = -1, lad_str_imin = -1;  // zero-ramp work skipped
   StepScalars* sc = nullptr;
   double* dtlog = nullptr;
   long long logcap = 1 << 20;
@@ -230,6 +231,7 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   }
   std::vector<double> dzl, drl, ramp;
   lad_tables(c->s, nz, nr, dzl, drl, ramp);
+  lad_skip_bounds(ramp.data(), nz, c->lad_seg_first, c->lad_src_imin, c->lad_str_imin);
   struct V { double** d; const std::vector<double>* h; } vs[] = {
       {&c->zc, &c->s.zc}, {&c->rc, &c->s.rc}, {&c->dxidz, &c->s.dxidz}, {&c->ddxidxidz, &c->s.ddxidxidz},
       {&c->detadr, &c->s.detadr}, {&c->ddetadetadr, &c->s.ddetadetadr}, {&c->dzl, &dzl}, {&c->drl, &drl},
@@ -403,6 +405,9 @@ static LadArgs lad_args(ajet_ctx* c, const double* q) {
   l.n_bnd = c->n_lad_bnd;
   l.rgroups = c->lad_groups;
   l.nrgroups = c->n_lad_groups;
+  l.seg_first = c->lad_seg_first;
+  l.src_imin = c->lad_src_imin;
+  l.str_imin = c->lad_str_imin;
   l.g = c->g;
   l.gam = c->p.gam;
   // ModShockCapturing.f90:66, 72, 79: C_mu Re, C_beta Re, C_kappa Re Pr

@@ -92,6 +92,10 @@ struct LadArgs {
   const uint8_t* str_mask;  // per tile: the strain rate and dilatation are needed
   int nseg;
   uint32_t gen;             // the step's stamp: a block is flagged when its flag equals it
+  // where the ramp is exactly zero the coefficients are +-0 and only ever added, so that work is
+  // skipped: filter segments below seg_first, source tiles ending at or before column src_imin,
+  // strain tiles ending at or before str_imin (the buffer holds zeros there)
+  int seg_first, src_imin, str_imin;
 };
 
 hipError_t upload_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
@@ -101,6 +105,7 @@ hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], c
 hipError_t launch_lad(const LadArgs& a, hipStream_t s, hipStream_t aux = nullptr, hipEvent_t fork = nullptr,
                       hipEvent_t join = nullptr);
 int lad_zfilter_segment(int nz);
+void lad_skip_bounds(const double* ramp, int nz, int& seg_first, int& src_imin, int& str_imin);
 // the dilatation sensor of the learned mode: flags[group(j) * nseg + i / seg] = gen where
 // max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min; then the tile masks of the
 // blocks so flagged

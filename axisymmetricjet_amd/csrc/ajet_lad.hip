@@ -33,6 +33,7 @@ __global__ __launch_bounds__(256) void lad_strain_kernel(LadArgs a, const int* _
   if (a.str_mask && !a.str_mask[t]) continue;
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
+  if (i1 <= a.str_imin) continue;  // only feeds zero-ramp columns
   for (int k = threadIdx.x; k < WZ * WR; k += blockDim.x) {
     const int wz = k % WZ, wr = k / WZ;
     const int i = i0 - HALO + wz, j = j0 - HALO + wr;
@@ -117,6 +118,7 @@ __global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* _
   if (a.src_mask && !a.src_mask[t]) continue;
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
+  if (i1 <= a.src_imin) continue;  // only feeds zero-ramp columns
   if (!INT) {
     // codes of the points where a second derivative is taken; outside the grid it is zero,
     // which the blank code gives
@@ -265,11 +267,12 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
-  const int S0 = (int)blockIdx.x * seg;
+  const int segi = (int)blockIdx.x + a.seg_first;  // segments before seg_first have zero ramps
+  if (a.zr_flags && a.zr_flags[segi + a.nseg * (int)blockIdx.y] != a.gen) return;
+  const int S0 = segi * seg;
   const int S1 = min(nz, S0 + seg);
   const int jb = a.rgroups[3 * blockIdx.y];
   const int out_lo = a.rgroups[3 * blockIdx.y + 1], out_hi = a.rgroups[3 * blockIdx.y + 2];
-  if (a.zr_flags && a.zr_flags[blockIdx.x + (unsigned)a.nseg * blockIdx.y] != a.gen) return;
   const int f = (int)blockIdx.z;
   const double* __restrict__ x = a.B + f * fs;
   double* __restrict__ out = a.out + f * fs;
@@ -512,6 +515,26 @@ hipError_t launch_lad_count(const uint32_t* flags, int n, uint32_t gen, int* cou
 
 int lad_zfilter_segment(int nz) { return zfilter_segment(nz); }
 
+// The work the exactly-zero ramp makes unnecessary (the ramp rises with z).  A filter segment
+// whose columns all have ramp 0 writes only +-0; the first segment that does not starts its
+// walk ZWARM columns early and reads the sources from 16 columns before that; a source tile
+// reads the strain rate up to LH = 7 columns beyond itself and spans at most TZ columns.
+void lad_skip_bounds(const double* ramp, int nz, int& seg_first, int& src_imin, int& str_imin) {
+  int ifirst = nz;
+  for (int i = 0; i < nz; ++i)
+    if (ramp[i] != 0.0) {
+      ifirst = i;
+      break;
+    }
+  const int seg = zfilter_segment(nz);
+  const int nseg = (nz + seg - 1) / seg;
+  seg_first = 0;
+  while (seg_first < nseg && min(nz, (seg_first + 1) * seg) <= ifirst) ++seg_first;
+  const int ib = max(0, seg_first * seg - ZWARM);
+  src_imin = ib - ZCH;
+  str_imin = src_imin - TZ - LH - 1;
+}
+
 hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, const int* line_group, double thr, double ramp_min,
                              hipStream_t s) {
   const long long n = (long long)a.g.nz * a.g.nr;
@@ -560,7 +583,9 @@ hipError_t launch_lad(const LadArgs& a, hipStream_t s, hipStream_t aux, hipEvent
   if ((e = pair(lad_strain_kernel<true>, lad_strain_kernel<false>)) != hipSuccess) return e;
   if ((e = pair(lad_source_kernel<true>, lad_source_kernel<false>)) != hipSuccess) return e;
   const int seg = zfilter_segment(g.nz);
-  hipLaunchKernelGGL(lad_zrfilter_kernel, dim3((g.nz + seg - 1) / seg, a.nrgroups, 3), dim3(ZLINES), 0, s, a, seg);
+  const int nseg = (g.nz + seg - 1) / seg;
+  if (nseg > a.seg_first)
+    hipLaunchKernelGGL(lad_zrfilter_kernel, dim3(nseg - a.seg_first, a.nrgroups, 3), dim3(ZLINES), 0, s, a, seg);
   return hipGetLastError();
 }
 
