@@ -120,10 +120,12 @@ def uniform_z(nz: int, z_max: float = 30.0) -> np.ndarray:
 
 
 def jet_profile(r: np.ndarray, mach: float = 1.5, theta: float = 0.05, tj_tinf: float = TJ_TINF,
-                gam: float = GAMMA):
+                gam: float = GAMMA, npr: float = 1.0):
     """Top-hat jet with Crocco-Busemann density (write_soln_file.f90:34-77 construction).
 
-    Returns rho, Vz, T on the given radii (T = 1/rho, so p = rho T = 1).
+    Returns rho, Vz, T on the given radii.  npr = 1 (the reference generator's jet): T = 1/rho,
+    so p = rho T = 1.  npr > 1: an under-expanded jet, p = 1 + (npr - 1) u(r) with the same top
+    hat u and the same T, rho = p / T (exactly the npr = 1 profile when npr = 1).
     """
     u = 0.5 * (1.0 + np.tanh((0.5 - r) / (2.0 * theta)))
     uj = mach * np.sqrt(tj_tinf)
@@ -132,7 +134,9 @@ def jet_profile(r: np.ndarray, mach: float = 1.5, theta: float = 0.05, tj_tinf: 
     term3 = (1.0 / tj_tinf) * (1.0 - u)
     rho_rhoj = 1.0 / (term1 + term2 + term3)
     rho = rho_rhoj / tj_tinf
-    return rho, u * uj, 1.0 / rho
+    t = 1.0 / rho
+    p = 1.0 + (npr - 1.0) * u
+    return rho * p, u * uj, t
 
 
 DEFAULT_BC = [
@@ -145,14 +149,15 @@ DEFAULT_BC = [
 
 def make_jet(nz: int, nr: int, seed: int = 1234, mach: float = 1.5, noise: float = 1e-3,
              theta: float = 0.05, re: float = 1.0e4, pr: float = 0.7,
-             bc_lines=None, r_max: float = 8.0, swirl: float = 0.0) -> JetCase:
+             bc_lines=None, r_max: float = 8.0, swirl: float = 0.0, npr: float = 1.0) -> JetCase:
     """The jet case on an nz x nr grid.  swirl > 0 adds seeded noise to Vtheta as well (a
-    test variant: the configured cases keep Vtheta = 0)."""
+    test variant: the configured cases keep Vtheta = 0); npr > 1 makes the jet under-expanded
+    (jet static pressure npr times the ambient), which forms shock cells."""
     zc = uniform_z(nz)
     rc = stretched_r(nr, r_max=r_max)
     z = np.asfortranarray(np.repeat(zc[:, None], nr, axis=1))
     r = np.asfortranarray(np.repeat(rc[None, :], nz, axis=0))
-    rho, vz, t = jet_profile(rc, mach=mach, theta=theta)
+    rho, vz, t = jet_profile(rc, mach=mach, theta=theta, npr=npr)
     target = np.zeros((nz, nr, 5), dtype=np.float64, order="F")
     target[:, :, 0] = rho[None, :]
     target[:, :, 3] = vz[None, :]

@@ -45,3 +45,26 @@ def test_learned_mode_acoustics_beat_the_method_without_the_network(case):
     assert lr["oaspl_err_db_max"] <= 0.5
     assert lr["oaspl_err_db_mean"] < zr["oaspl_err_db_mean"]
     assert lr["spectrum_err_db"] < zr["spectrum_err_db"]
+
+
+def test_sensor_fires_on_the_shock_cells_of_an_under_expanded_jet():
+    """Jet static pressure 1.5 times the ambient: after 750 steps under the reference LAD the
+    jet has formed compressions; the learned mode sends a few of its filter blocks (those
+    holding them) back to the reference LAD, and none for the ideally expanded jet."""
+    shocked = jc.make_jet(4400, 640, seed=1234, npr=1.5)
+    ctx = capi.Context(shocked, **ll.LAD)
+    ctx.run(750)
+    q = ctx.state()
+    model = ll.load()
+    lm = capi.Context(shocked, **ll.LAD)
+    lm.set_state(q, shocked.target)
+    lm.set_learned_lad(model, sensor_threshold=0.05, ramp_min=1e-8, refresh=4)
+    lm.run(4)
+    frac = lm.fallback_fraction()
+    print(f"under-expanded jet: fallback fraction {frac:.4f}", flush=True)
+    assert 0.0 < frac < 0.1
+    assert np.all(np.isfinite(lm.state()))
+    plain = capi.Context(jc.make_jet(4400, 640, seed=1234), **ll.LAD)
+    plain.set_learned_lad(model, sensor_threshold=0.05, ramp_min=1e-8, refresh=4)
+    plain.run(4)
+    assert plain.fallback_fraction() == 0.0
