@@ -10,7 +10,11 @@
 set -euo pipefail
 REF=${REF:-/root/reference}
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+# REF_OPT: the optimisation level; -O0 (default) is the reference's own build and the one the
+# fixtures come from, another level (e.g. -O2, for timing only) goes to oracle/_ref/opt<level>/
+REF_OPT=${REF_OPT:--O0}
 OUT="$HERE/_ref"
+if [ "$REF_OPT" != "-O0" ]; then OUT="$HERE/_ref/opt${REF_OPT#-}"; fi
 BLD="$OUT/build"
 MPI_PREFIX=${MPI_PREFIX:-/opt/conda}
 FC=${FC:-amdflang}
@@ -20,7 +24,7 @@ if [ ! -d "$REF" ]; then
   exit 0
 fi
 mkdir -p "$BLD"
-FFLAGS="-O0 -ffp-contract=off -I$MPI_PREFIX/include -module-dir $BLD"
+FFLAGS="$REF_OPT -ffp-contract=off -I$MPI_PREFIX/include -module-dir $BLD"
 # dependency order of the modules PROGRAM ModSolver uses (run_compile.sh:5-7 lists the files)
 MODS="ModuleVariables ModPLOT3D_IO ModDecomp ModCommunicate ModInputOutput ModAllocate ModAssign \
 ModOperator ModDeriv ModTGfilter ModShockCapturing ModControlForcing ModBC ModPatch ModAxis ModRHS \

@@ -25,7 +25,7 @@ sys.path.insert(0, HERE)
 
 from axisymmetricjet_amd import jetcase as jc  # noqa: E402
 
-HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+HARNESS = os.environ.get("REF_HARNESS") or os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 
 from cases import LAD_C, cases, feedback_forcing, input_digest  # noqa: E402
 
