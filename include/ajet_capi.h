@@ -222,6 +222,7 @@ int ajet_lad_fallback_fraction(ajet_ctx* ctx, double* fraction);
  * sum((y - t)^2) / (2 batch), gradients through every layer with BF16 operands and FP32
  * accumulation, torch.optim.Adam's update (beta 0.9, 0.999, eps 1e-8) on FP32 masters. */
 typedef struct ajet_trainer ajet_trainer;
+/* On failure *out may still hold a trainer (for ajet_trainer_last_error): destroy it. */
 int ajet_trainer_create(int32_t batch, const ajet_mlp* init, ajet_trainer** out);
 /* samples: x (n x 104, normalised as the model's shift and scale say) and targets t (n x 2) */
 int ajet_trainer_load(ajet_trainer* tr, const float* x, const float* t, int64_t n);
