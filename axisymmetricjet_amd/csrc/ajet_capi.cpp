@@ -33,8 +33,10 @@ struct ajet_ctx {
   int *zcuts = nullptr, *rcuts = nullptr;
   int *int_tiles = nullptr, *bnd_tiles = nullptr;  // interior and boundary tile lists
   int n_int = 0, n_bnd = 0;
-  int *lad_int = nullptr, *lad_bnd = nullptr;      // the same split for the LAD kernels
-  int n_lad_int = 0, n_lad_bnd = 0;
+  int* mix_tiles = nullptr;  // both in one list (mixed_tile_list), for the LAD-mode stage kernel
+  int n_mix = 0;
+  int* lad_tiles = nullptr;  // the strain and source kernels' tiles, split the same way
+  int n_lad_tiles = 0;
   int* lad_groups = nullptr;                       // line groups of the fused z/r filter
   int n_lad_groups = 0;
   int lad_seg_first = 0, lad_src_imin = -1, lad_str_imin = -1;  // zero-ramp work skipped
@@ -141,7 +143,7 @@ int ajet_destroy(ajet_ctx* c) {
   void* bufs[] = {c->Q0, c->Q1, c->Q2, c->A, c->S, c->tgt, c->lad, c->ladA, c->ladB, c->dzl, c->drl, c->ramp,
                   c->meta, c->zc, c->rc, c->dxidz,
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
-                  c->int_tiles, c->bnd_tiles, c->lad_int, c->lad_bnd, c->lad_groups, c->codes, c->sc, c->dtlog,
+                  c->int_tiles, c->bnd_tiles, c->mix_tiles, c->lad_tiles, c->lad_groups, c->codes, c->sc, c->dtlog,
                   c->mlpW, c->mlpB, c->mlpShift, c->mlpScale, c->zr_flags, c->src_mask, c->str_mask,
                   c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out, c->count_dev};
   if (c->aux) (void)hipStreamSynchronize(c->aux);
@@ -267,15 +269,19 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
       HIPCHK(c, hipMemcpy(c->int_tiles, it.data(), sizeof(int) * it.size(), hipMemcpyHostToDevice));
     if (!bt.empty())
       HIPCHK(c, hipMemcpy(c->bnd_tiles, bt.data(), sizeof(int) * bt.size(), hipMemcpyHostToDevice));
+    // the step scalars are written by the first boundary tile of a launch
+    if (bt.empty()) return fail(c, AJET_EINVAL, "the grid has no boundary tile");
+    std::vector<int> mix;
+    mixed_tile_list(it, bt, mix);
+    c->n_mix = (int)mix.size();
+    HIPCHK(c, hipMalloc((void**)&c->mix_tiles, sizeof(int) * mix.size()));
+    HIPCHK(c, hipMemcpy(c->mix_tiles, mix.data(), sizeof(int) * mix.size(), hipMemcpyHostToDevice));
     classify_lad_tiles(c->s, nz, nr, it, bt);
-    c->n_lad_int = (int)it.size();
-    c->n_lad_bnd = (int)bt.size();
-    HIPCHK(c, hipMalloc((void**)&c->lad_int, sizeof(int) * (it.size() + 1)));
-    HIPCHK(c, hipMalloc((void**)&c->lad_bnd, sizeof(int) * (bt.size() + 1)));
-    if (!it.empty())
-      HIPCHK(c, hipMemcpy(c->lad_int, it.data(), sizeof(int) * it.size(), hipMemcpyHostToDevice));
-    if (!bt.empty())
-      HIPCHK(c, hipMemcpy(c->lad_bnd, bt.data(), sizeof(int) * bt.size(), hipMemcpyHostToDevice));
+    mixed_tile_list(it, bt, mix);
+    c->n_lad_tiles = (int)mix.size();
+    HIPCHK(c, hipMalloc((void**)&c->lad_tiles, sizeof(int) * (mix.size() + 1)));
+    if (!mix.empty())
+      HIPCHK(c, hipMemcpy(c->lad_tiles, mix.data(), sizeof(int) * mix.size(), hipMemcpyHostToDevice));
     std::vector<int> gr;
     build_lad_line_groups(c->s, nz, nr, LAD_LINES, gr);
     c->n_lad_groups = (int)gr.size() / 3;
@@ -399,10 +405,8 @@ static LadArgs lad_args(ajet_ctx* c, const double* q) {
   l.out = c->lad;
   l.zcuts = c->zcuts;
   l.rcuts = c->rcuts;
-  l.int_tiles = c->lad_int;
-  l.bnd_tiles = c->lad_bnd;
-  l.n_int = c->n_lad_int;
-  l.n_bnd = c->n_lad_bnd;
+  l.tiles = c->lad_tiles;
+  l.ntiles = c->n_lad_tiles;
   l.rgroups = c->lad_groups;
   l.nrgroups = c->n_lad_groups;
   l.seg_first = c->lad_seg_first;
@@ -465,8 +469,7 @@ static hipError_t learned_after_step(ajet_ctx* c, hipStream_t s) {
   return e;
 }
 
-static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStream_t s, hipStream_t aux,
-                                 hipEvent_t fork, hipEvent_t join) {
+static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStream_t s) {
   hipError_t e;
   if (stage == 1 && c->learned_steps % c->refresh == 0 &&
       (e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, s)) != hipSuccess)
@@ -486,23 +489,25 @@ static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStr
   l.zr_flags = c->zr_flags;
   l.src_mask = c->src_mask;
   l.str_mask = c->str_mask;
-  return launch_lad(l, s, aux, fork, join);
+  return launch_lad(l, s);
 }
 
 // The artificial coefficients of one stage of a run in LAD mode (the reference's or learned).
-static hipError_t launch_coefficients(ajet_ctx* c, int stage, const double* q, hipStream_t s, hipStream_t aux,
-                                      hipEvent_t fork, hipEvent_t join) {
-  return c->learned ? launch_learned(c, stage, q, s, aux, fork, join) : launch_lad(lad_args(c, q), s, aux, fork, join);
+static hipError_t launch_coefficients(ajet_ctx* c, int stage, const double* q, hipStream_t s) {
+  return c->learned ? launch_learned(c, stage, q, s) : launch_lad(lad_args(c, q), s);
 }
 
 static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
   const int mode = physics_mode(c->p);
   hipError_t e;
-  // the LAD launches stay on one stream: a join across streams costs ~10 us of idle GPU between
-  // dependent kernels, more than the boundary tiles' strain and source launches take
-  if (mode == MODE_LAD && (e = launch_coefficients(c, stage, a.qin, c->stream, nullptr, nullptr, nullptr)) !=
-                              hipSuccess)
-    return e;
+  if (mode == MODE_LAD && (e = launch_coefficients(c, stage, a.qin, c->stream)) != hipSuccess) return e;
+  if (mode == MODE_LAD && stage >= 1 && stage <= 3) {
+    // one launch for both tile classes: an event between two streams costs ~6 us of idle GPU
+    // each way, and the stage kernel holds 2 waves per SIMD here for either class
+    a.tiles = c->mix_tiles;
+    a.ntiles = c->n_mix;
+    return launch_stage_mixed(stage, a, c->stream);
+  }
   if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
   a.tiles = c->bnd_tiles;
@@ -920,7 +925,7 @@ static hipError_t batch_stage(ajet_batch* b, int stage, int dt_from_dmax, int pa
   if (b->mode == MODE_LAD) {
     for (ajet_ctx* c : b->runs) {
       const double* q = stage == 1 ? c->Q0 : (stage == 3 ? c->Q2 : c->Q1);
-      if ((e = launch_coefficients(c, stage, q, b->stream, nullptr, nullptr, nullptr)) != hipSuccess) return e;
+      if ((e = launch_coefficients(c, stage, q, b->stream)) != hipSuccess) return e;
     }
   }
   if ((e = hipEventRecord(b->ev_fork, b->stream)) != hipSuccess) return e;

@@ -90,6 +90,7 @@ int build_setup(const ajet_params& p, const double* z, const double* r, const in
 int build_tiling(Setup& s, int nz, int nr, std::string& err);
 void classify_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, std::vector<int>& boundary);
 void classify_lad_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, std::vector<int>& boundary);
+void mixed_tile_list(const std::vector<int>& interior, const std::vector<int>& boundary, std::vector<int>& out);
 void build_lad_line_groups(const Setup& s, int nz, int nr, int lines, std::vector<int>& groups);
 void lad_tables(const Setup& s, int nz, int nr, std::vector<double>& dz, std::vector<double>& dr,
                 std::vector<double>& ramp);

@@ -77,9 +77,8 @@ struct LadArgs {
   double* out;             // 3 fields: mu*, beta*, kappa*
   const int* zcuts;
   const int* rcuts;
-  const int* int_tiles;    // tiles whose points and stencil rows are all interior
-  const int* bnd_tiles;    // the others
-  int n_int, n_bnd;
+  const int* tiles;        // strain and source tiles: t interior (all points and stencil rows
+  int ntiles;              // interior), ~t boundary, in the order of mixed_tile_list
   const int* rgroups;      // line groups of the fused z/r filter: (wb, out_lo, out_hi) each
   int nrgroups;
   Geometry g;
@@ -100,10 +99,8 @@ struct LadArgs {
 
 hipError_t upload_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
 hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
-// The four LAD launches on stream s; with aux (and two events) the boundary-tile strain and
-// source launches run on aux beside the interior ones.
-hipError_t launch_lad(const LadArgs& a, hipStream_t s, hipStream_t aux = nullptr, hipEvent_t fork = nullptr,
-                      hipEvent_t join = nullptr);
+// The three LAD launches (strain, source, fused z/r filter) on stream s.
+hipError_t launch_lad(const LadArgs& a, hipStream_t s);
 int lad_zfilter_segment(int nz);
 void lad_skip_bounds(const double* ramp, int nz, int& seg_first, int& src_imin, int& str_imin);
 // the dilatation sensor of the learned mode: flags[group(j) * nseg + i / seg] = gen where
@@ -189,6 +186,9 @@ hipError_t launch_mlp_infer(const MlpInfer& a, int nblocks, hipStream_t s);
 enum { MODE_PLAIN = 0, MODE_SC = 1, MODE_LAD = 2, MODE_INVISCID = 3 };
 
 hipError_t launch_stage(int stage, int mode, bool interior, const StageArgs& a, hipStream_t s);
+// Stages 1-3 of the LAD mode with the interior and boundary tiles in one launch: a.tiles holds
+// the list of mixed_tile_list (t interior, ~t boundary, a boundary tile first).
+hipError_t launch_stage_mixed(int stage, const StageArgs& a, hipStream_t s);
 // Batched stages (MODE_PLAIN and MODE_INVISCID): `runs` and `list` in device memory, list
 // entries = tile | run << 20 | writer << 31.
 hipError_t launch_stage_batch(int stage, int mode, bool interior, bool force, const StageArgs* runs,

@@ -18,22 +18,17 @@ namespace ajet {
 // ---- |S|, dil, f_sw (ComputeDilatation, ComputeMgnStrRate ModShockCapturing.f90:108-159, and
 // the switch of ComputeTobeFilteredQuantities :196-202).  Same tiles and 2-point window as the
 // stage kernel; the first derivatives take the patch closures and are zero at blanked points.
-// INT: the tile list holds tiles whose points and stencil rows are all interior (see
-// classify_lad_tiles), so no metadata is read.
+// INT: tile t's points and stencil rows are all interior (see classify_lad_tiles), so no
+// metadata is read.
 template <bool INT>
-__global__ __launch_bounds__(256) void lad_strain_kernel(LadArgs a, const int* __restrict__ tiles, int ntiles) {
-  __shared__ double sVr[WR][WZ];
-  __shared__ double sVz[WR][WZ];
+__device__ __forceinline__ void strain_tile(const LadArgs& a, int t, double (&sVr)[WR][WZ], double (&sVz)[WR][WZ]) {
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
-  // one tile per block, or (a grid smaller than the list) several in turn
-  for (int li = xcd_tile((int)blockIdx.x, (int)gridDim.x); li < ntiles; li += (int)gridDim.x) {
-  const int t = tiles[li];
-  if (a.str_mask && !a.str_mask[t]) continue;
+  if (a.str_mask && !a.str_mask[t]) return;
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
-  if (i1 <= a.str_imin) continue;  // only feeds zero-ramp columns
+  if (i1 <= a.str_imin) return;  // only feeds zero-ramp columns
   for (int k = threadIdx.x; k < WZ * WR; k += blockDim.x) {
     const int wz = k % WZ, wr = k / WZ;
     const int i = i0 - HALO + wz, j = j0 - HALO + wr;
@@ -85,6 +80,17 @@ __global__ __launch_bounds__(256) void lad_strain_kernel(LadArgs a, const int* _
     a.A[gi + 2 * fs] = fsw;
   }
   __syncthreads();  // the window is reused by the next tile
+}
+
+// Interior and boundary tiles in one launch: list entry t >= 0 is interior tile t, ~t boundary
+// tile t.  One tile per block, or (a grid smaller than the list) several in turn.
+__global__ __launch_bounds__(256) void lad_strain_kernel(LadArgs a) {
+  __shared__ double sVr[WR][WZ];
+  __shared__ double sVz[WR][WZ];
+  for (int li = xcd_tile((int)blockIdx.x, (int)gridDim.x); li < a.ntiles; li += (int)gridDim.x) {
+    const int e = a.tiles[li];
+    if (e >= 0) strain_tile<true>(a, e, sVr, sVz);
+    else strain_tile<false>(a, ~e, sVr, sVz);
   }
 }
 
@@ -100,25 +106,30 @@ constexpr int LWR = TR + 2 * LH;
 // The three fields pass through one LDS window in turn (|S|, dil, then T): 22 KB per block
 // instead of 62 KB for all three at once, so about seven blocks share a CU instead of two; each
 // point keeps its three pairs of fourth derivatives in registers across the passes.
+struct SourceLds {
+  double sB[LWR][LWZ];     // the field of this pass
+  double sZ[TR][TZ + 4];   // its D2 along xi, columns i0-2 .. i0+TZ+1
+  double sR[TR + 4][TZ];   // its D2 along eta, rows j0-2 .. j0+TR+1
+  uint8_t cZ[TR][TZ + 4];  // z row codes where the D2 along xi is taken
+  uint8_t cR[TR + 4][TZ];  // r row codes where the D2 along eta is taken
+};
+
 template <bool INT>
-__global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* __restrict__ tiles, int ntiles) {
-  __shared__ double sB[LWR][LWZ];     // the field of this pass
-  __shared__ double sZ[TR][TZ + 4];   // its D2 along xi, columns i0-2 .. i0+TZ+1
-  __shared__ double sR[TR + 4][TZ];   // its D2 along eta, rows j0-2 .. j0+TR+1
-  __shared__ uint8_t cZ[TR][TZ + 4];  // z row codes where the D2 along xi is taken
-  __shared__ uint8_t cR[TR + 4][TZ];  // r row codes where the D2 along eta is taken
+__device__ __forceinline__ void source_tile(const LadArgs& a, int t, SourceLds& sh) {
+  auto& sB = sh.sB;
+  auto& sZ = sh.sZ;
+  auto& sR = sh.sR;
+  auto& cZ = sh.cZ;
+  auto& cR = sh.cR;
   constexpr int ROWS = 256 / TZ;                 // rows of owned points per pass of the block
   constexpr int NP = (TR + ROWS - 1) / ROWS;     // owned points per thread
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
-  // one tile per block, or (a grid smaller than the list) several in turn
-  for (int li = xcd_tile((int)blockIdx.x, (int)gridDim.x); li < ntiles; li += (int)gridDim.x) {
-  const int t = tiles[li];
-  if (a.src_mask && !a.src_mask[t]) continue;
+  if (a.src_mask && !a.src_mask[t]) return;
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
-  if (i1 <= a.src_imin) continue;  // only feeds zero-ramp columns
+  if (i1 <= a.src_imin) return;  // only feeds zero-ramp columns
   if (!INT) {
     // codes of the points where a second derivative is taken; outside the grid it is zero,
     // which the blank code gives
@@ -188,6 +199,15 @@ __global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a, const int* _
     a.B[gi + 2 * fs] = rho / (a.gam * sqrt(T)) * fabs(dz4[2][p] * dz + dr4[2][p] * dr);
   }
   __syncthreads();  // the window and codes are reused by the next tile
+}
+
+// Interior and boundary tiles in one launch, as the strain kernel.
+__global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a) {
+  __shared__ SourceLds sh;
+  for (int li = xcd_tile((int)blockIdx.x, (int)gridDim.x); li < a.ntiles; li += (int)gridDim.x) {
+    const int e = a.tiles[li];
+    if (e >= 0) source_tile<true>(a, e, sh);
+    else source_tile<false>(a, ~e, sh);
   }
 }
 
@@ -558,30 +578,19 @@ hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], c
   return e;
 }
 
-hipError_t launch_lad(const LadArgs& a, hipStream_t s, hipStream_t aux, hipEvent_t fork, hipEvent_t join) {
+hipError_t launch_lad(const LadArgs& a, hipStream_t s) {
   const Geometry& g = a.g;
   // masked (learned mode): most tiles return at once, so a grid of at most MASKED_GRID blocks
   // walks the list instead of one block per tile
   constexpr int MASKED_GRID = 1024;
   const bool masked = a.src_mask != nullptr;
   auto grid = [&](int n) { return dim3((unsigned)(((masked && n > MASKED_GRID ? MASKED_GRID : n) + 7) / 8 * 8)); };
-  hipError_t e;
-  // boundary tiles on the second stream beside the interior tiles (when one is given); the
-  // sources read the strain rate of neighbouring tiles, so each kernel pair joins before the next
-  auto pair = [&](auto kint, auto kbnd) -> hipError_t {
-    hipStream_t sb = aux ? aux : s;
-    if (aux && a.n_bnd > 0) {
-      if ((e = hipEventRecord(fork, s)) != hipSuccess) return e;
-      if ((e = hipStreamWaitEvent(aux, fork, 0)) != hipSuccess) return e;
-    }
-    if (a.n_bnd > 0) hipLaunchKernelGGL(kbnd, grid(a.n_bnd), dim3(256), 0, sb, a, a.bnd_tiles, a.n_bnd);
-    if (aux && a.n_bnd > 0 && (e = hipEventRecord(join, aux)) != hipSuccess) return e;
-    if (a.n_int > 0) hipLaunchKernelGGL(kint, grid(a.n_int), dim3(256), 0, s, a, a.int_tiles, a.n_int);
-    if (aux && a.n_bnd > 0 && (e = hipStreamWaitEvent(s, join, 0)) != hipSuccess) return e;
-    return hipGetLastError();
-  };
-  if ((e = pair(lad_strain_kernel<true>, lad_strain_kernel<false>)) != hipSuccess) return e;
-  if ((e = pair(lad_source_kernel<true>, lad_source_kernel<false>)) != hipSuccess) return e;
+  // the interior and the boundary tiles of each kernel in one launch: a boundary tile's launch
+  // of its own took 10-25 us, most of it latency with a few hundred blocks on 256 CUs
+  if (a.ntiles > 0) {
+    hipLaunchKernelGGL(lad_strain_kernel, grid(a.ntiles), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(lad_source_kernel, grid(a.ntiles), dim3(256), 0, s, a);
+  }
   const int seg = zfilter_segment(g.nz);
   const int nseg = (g.nz + seg - 1) / seg;
   if (nseg > a.seg_first)

@@ -307,6 +307,29 @@ void classify_lad_tiles(const Setup& s, int nz, int nr, std::vector<int>& interi
     }
 }
 
+// One launch list for interior and boundary tiles: interior tile t as t, boundary tile t as ~t.
+// A launch of pad8(n) blocks gives XCD x (blocks x, x + 8, ...) the list positions
+// [x C, (x + 1) C), C = pad8(n) / 8, in order.  Each XCD's range starts with its share of the
+// boundary tiles (the slower ones, first) and goes on with a contiguous run of interior tiles,
+// so the XCDs hold about the same work and neighbouring interior tiles share an L2.  Position 0
+// holds a boundary tile when there is one.
+void mixed_tile_list(const std::vector<int>& interior, const std::vector<int>& boundary, std::vector<int>& out) {
+  const int nb = (int)boundary.size(), ni = (int)interior.size(), n = nb + ni;
+  const int C = (n + 7) / 8;
+  out.clear();
+  out.reserve(n);
+  int bi = 0, ii = 0;
+  for (int x = 0; x < 8; ++x) {
+    const int size = std::max(0, std::min(C, n - x * C));
+    const int share = (nb + 7 - x) / 8;  // the eight shares sum to nb, the first ones larger
+    const int kb = std::min(std::min(share, size), nb - bi);
+    for (int k = 0; k < kb; ++k) out.push_back(~boundary[bi++]);
+    const int ki = std::min(size - kb, ni - ii);
+    for (int k = 0; k < ki; ++k) out.push_back(interior[ii++]);
+    for (int k = kb + ki; k < size; ++k) out.push_back(~boundary[bi++]);  // interior ran out
+  }
+}
+
 // Line groups of the fused z/r filter: each group walks `lines` consecutive r-lines from wb
 // through the z recursion and writes the r-filtered output of lines [out_lo, out_hi).  The r
 // filter of line j reads lines j-4..j+4 (interior rows) or its closure row's 11 lines, clipped

@@ -98,3 +98,15 @@ def test_tuned_tile_is_a_candidate_and_reaches_the_build():
     # every timed candidate is listed, and the choice is the fastest on the stated metric
     t = doc["timings_ms_per_step"]
     assert min(t, key=lambda k: t[k]["lad"]) == "x".join(str(x) for x in doc["tile"])
+
+
+def test_mixed_tile_list(tmp_path):
+    """The launch list that puts interior and boundary tiles in one launch (host C++, built here
+    with g++): each tile once, boundary tiles marked, a boundary tile first (it writes the step
+    scalars), each XCD's range starting with its share of the boundary tiles."""
+    exe = str(tmp_path / "tile_lists_check")
+    subprocess.check_call(["g++", "-O1", "-std=c++17", "-ffunction-sections", "-fdata-sections", "-Wl,--gc-sections",
+                           "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "tile_lists_check.cpp"),
+                           os.path.join(ROOT, "axisymmetricjet_amd", "csrc", "ajet_setup.cpp"), "-o", exe])
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
