@@ -3,6 +3,7 @@
 // temperature of the CFL rule and the accumulated time live on the device.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -61,9 +62,10 @@ struct ajet_ctx {
   uint32_t gen = 0;                        // stamp of the sensor's flags (one per step)
   bool sensor_ready = false;               // the sensor of the current state ran after the last step
   bool skip_lad = false;                   // this step flagged no block: no LAD launches
-  int* count_dev = nullptr;                // flagged blocks, on the device
-  int* count_host = nullptr;               // ... and in pinned host memory
-  hipEvent_t ev_sensor = nullptr;          // the count has arrived
+  uint32_t* any_dev = nullptr;             // the sensor's stamp when it flagged a block
+  uint32_t* sensor_host = nullptr;         // pinned, fine-grained: [0] stamp, [1] any block flagged
+  uint32_t* sensor_host_dev = nullptr;     // ... its device address
+  hipEvent_t ev_sensor = nullptr;          // the sensor's result is published
   std::vector<int> lad_groups_h;           // host copy of the line groups
   int nseg = 0;                            // z segments of the filter blocks
   uint32_t* zr_flags = nullptr;            // flagged filter blocks (segment + nseg * group)
@@ -145,11 +147,11 @@ int ajet_destroy(ajet_ctx* c) {
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
                   c->int_tiles, c->bnd_tiles, c->mix_tiles, c->lad_tiles, c->lad_groups, c->codes, c->sc, c->dtlog,
                   c->mlpW, c->mlpB, c->mlpShift, c->mlpScale, c->zr_flags, c->src_mask, c->str_mask,
-                  c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out, c->count_dev};
+                  c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out, c->any_dev};
   if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
-  if (c->count_host) (void)hipHostFree(c->count_host);
+  if (c->sensor_host) (void)hipHostFree(c->sensor_host);
   if (c->ev_sensor) (void)hipEventDestroy(c->ev_sensor);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
   if (c->ev_join) (void)hipEventDestroy(c->ev_join);
@@ -447,18 +449,42 @@ static MlpInfer mlp_args(ajet_ctx* c, const double* q) {
 // Learned mode: the network's coefficients at the first stage of every refresh-th step; the
 // dilatation sensor at the first stage of every step; at every stage the reference's LAD on the
 // filter blocks it flagged.
-// The dilatation sensor, the tile masks and the count of flagged blocks for the state q, under
-// a new stamp; the count goes to pinned host memory behind ev_sensor.
+// The dilatation sensor for the state q under a new stamp, and the publication of whether it
+// flagged a block to pinned host memory (ev_sensor behind it).
 static hipError_t learned_sensor(ajet_ctx* c, const double* q, hipStream_t s) {
   hipError_t e;
   LadArgs l = lad_args(c, q);
   l.nseg = c->nseg;
   l.gen = ++c->gen;
-  if ((e = launch_lad_sensor(l, c->zr_flags, c->line_group, c->sensor_thr, c->ramp_min, s)) != hipSuccess) return e;
-  if ((e = launch_lad_masks(l, c->zr_flags, c->src_mask, c->str_mask, s)) != hipSuccess) return e;
-  if ((e = launch_lad_count(c->zr_flags, c->nseg * c->n_lad_groups, c->gen, c->count_dev, s)) != hipSuccess) return e;
-  if ((e = hipMemcpyAsync(c->count_host, c->count_dev, sizeof(int), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = launch_lad_sensor(l, c->zr_flags, c->any_dev, c->line_group, c->sensor_thr, c->ramp_min, s)) !=
+      hipSuccess)
+    return e;
+  if ((e = launch_lad_publish(c->any_dev, c->gen, c->sensor_host_dev, s)) != hipSuccess) return e;
   return hipEventRecord(c->ev_sensor, s);
+}
+
+// Waits for the sensor result of the current stamp.  The host polls the pinned word the GPU
+// stores last: a blocking wait on the event took ~30 us to wake the host, GPU time left idle
+// each step.  After 200 ms of polling (a long queue ahead) it waits on the event instead, which
+// also reports a failed launch.
+static hipError_t wait_sensor(ajet_ctx* c, bool* any) {
+  const uint32_t gen = c->gen;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned it = 1;; ++it) {
+    if (__atomic_load_n(&c->sensor_host[0], __ATOMIC_ACQUIRE) == gen) {
+      *any = __atomic_load_n(&c->sensor_host[1], __ATOMIC_RELAXED) != 0u;
+      return hipSuccess;
+    }
+    if ((it & 1023u) == 0u && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) break;
+#if defined(__x86_64__)
+    __builtin_ia32_pause();
+#endif
+  }
+  const hipError_t e = hipEventSynchronize(c->ev_sensor);
+  if (e != hipSuccess) return e;
+  if (__atomic_load_n(&c->sensor_host[0], __ATOMIC_ACQUIRE) != gen) return hipErrorUnknown;
+  *any = __atomic_load_n(&c->sensor_host[1], __ATOMIC_RELAXED) != 0u;
+  return hipSuccess;
 }
 
 // After the last stage of a step: the sensor of the new state, so that the next step knows
@@ -475,17 +501,20 @@ static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStr
       (e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, s)) != hipSuccess)
     return e;
   if (stage == 4) ++c->learned_steps;
+  bool any = false;
   if (stage == 1) {
-    // the sensor of this state: taken after the previous step (wait for its count), or now
+    // the sensor of this state: taken after the previous step (wait for its result), or now
     if (!c->sensor_ready && (e = learned_sensor(c, q, s)) != hipSuccess) return e;
     c->sensor_ready = false;
-    if ((e = hipEventSynchronize(c->ev_sensor)) != hipSuccess) return e;
-    c->skip_lad = *c->count_host == 0;
+    if ((e = wait_sensor(c, &any)) != hipSuccess) return e;
+    c->skip_lad = !any;
   }
   if (c->skip_lad) return hipSuccess;  // no block flagged: the LAD launches would all return at once
   LadArgs l = lad_args(c, q);
   l.nseg = c->nseg;
-  l.gen = c->gen;
+  l.gen = c->gen;  // the stamp of this step's sensor
+  // the tile masks of the flagged blocks, for the four stages of this step
+  if (any && (e = launch_lad_masks(l, c->zr_flags, c->src_mask, c->str_mask, s)) != hipSuccess) return e;
   l.zr_flags = c->zr_flags;
   l.src_mask = c->src_mask;
   l.str_mask = c->str_mask;
@@ -713,9 +742,12 @@ int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_thres
     c->nseg = (g.nz + lad_zfilter_segment(g.nz) - 1) / lad_zfilter_segment(g.nz);
     HIPCHK(c, hipMalloc((void**)&c->zr_flags, sizeof(uint32_t) * (size_t)c->nseg * c->n_lad_groups));
     HIPCHK(c, hipMemset(c->zr_flags, 0, sizeof(uint32_t) * (size_t)c->nseg * c->n_lad_groups));
-    HIPCHK(c, hipMalloc((void**)&c->count_dev, sizeof(int)));
-    HIPCHK(c, hipHostMalloc((void**)&c->count_host, sizeof(int), hipHostMallocDefault));
-    *c->count_host = 0;
+    HIPCHK(c, hipMalloc((void**)&c->any_dev, sizeof(uint32_t)));
+    HIPCHK(c, hipMemset(c->any_dev, 0, sizeof(uint32_t)));
+    HIPCHK(c, hipHostMalloc((void**)&c->sensor_host, 2 * sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent));
+    c->sensor_host[0] = 0u;
+    c->sensor_host[1] = 0u;
+    HIPCHK(c, hipHostGetDevicePointer((void**)&c->sensor_host_dev, c->sensor_host, 0));
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_sensor, hipEventDisableTiming));
     HIPCHK(c, hipMalloc((void**)&c->src_mask, (size_t)g.ntz * g.ntr));
     HIPCHK(c, hipMalloc((void**)&c->str_mask, (size_t)g.ntz * g.ntr));

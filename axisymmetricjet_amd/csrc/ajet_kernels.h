@@ -104,13 +104,14 @@ hipError_t launch_lad(const LadArgs& a, hipStream_t s);
 int lad_zfilter_segment(int nz);
 void lad_skip_bounds(const double* ramp, int nz, int& seg_first, int& src_imin, int& str_imin);
 // the dilatation sensor of the learned mode: flags[group(j) * nseg + i / seg] = gen where
-// max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min; then the tile masks of the
-// blocks so flagged
-hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, const int* line_group, double thr, double ramp_min,
-                             hipStream_t s);
+// max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min, and *any = gen when it flags a
+// block; then the publication of its result to host-visible memory, host[1] = (*any == gen) and
+// host[0] = gen (stored last, with system-scope release); the tile masks of the flagged blocks
+hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, uint32_t* any, const int* line_group, double thr,
+                             double ramp_min, hipStream_t s);
+hipError_t launch_lad_publish(const uint32_t* any, uint32_t gen, uint32_t* host, hipStream_t s);
 hipError_t launch_lad_masks(const LadArgs& a, const uint32_t* flags, uint8_t* src_mask, uint8_t* str_mask,
                             hipStream_t s);
-hipError_t launch_lad_count(const uint32_t* flags, int n, uint32_t gen, int* count, hipStream_t s);
 
 // ---- learned LAD coefficient (ajet_mlp.hip): 104 features -> 128 -> 128 -> 128 -> 2.
 constexpr int MLP_NIN = 104;       // features

@@ -461,6 +461,7 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
 
 // ---- learned mode: the dilatation sensor and the tile masks of the flagged filter blocks.
 __global__ __launch_bounds__(256) void lad_sensor_kernel(LadArgs a, uint32_t* __restrict__ flags,
+                                                         uint32_t* __restrict__ any,
                                                          const int* __restrict__ line_group, int seg, double thr,
                                                          double ramp_min) {
   const Geometry& g = a.g;
@@ -481,7 +482,16 @@ __global__ __launch_bounds__(256) void lad_sensor_kernel(LadArgs a, uint32_t* __
     // the step's stamp: every writer stores the same value, and older stamps need no clearing
     uint32_t* f = &flags[line_group[j] * a.nseg + i / seg];
     if (*f != a.gen) *f = a.gen;
+    if (*any != a.gen) *any = a.gen;
   }
+}
+
+// One thread: whether the sensor flagged any block, then the stamp, into host memory that the
+// host polls (fine-grained, so the stores reach it without a cache flush).
+__global__ void lad_publish_kernel(const uint32_t* __restrict__ any, uint32_t gen, uint32_t* host) {
+  if (threadIdx.x != 0) return;
+  __hip_atomic_store(&host[1], *any == gen ? 1u : 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  __hip_atomic_store(&host[0], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // A flagged block (segment sg, group gr) reads the sources over columns [ib - 16, S1 + 25)
@@ -513,23 +523,8 @@ __global__ __launch_bounds__(256) void lad_mask_kernel(LadArgs a, const uint32_t
   str_mask[t] = str ? 1 : 0;
 }
 
-// the number of flagged blocks (flags equal to gen), one block of 256 threads
-__global__ __launch_bounds__(256) void lad_count_kernel(const uint32_t* __restrict__ flags, int n, uint32_t gen,
-                                                        int* __restrict__ count) {
-  __shared__ int part[256];
-  int c = 0;
-  for (int k = (int)threadIdx.x; k < n; k += 256) c += flags[k] == gen ? 1 : 0;
-  part[threadIdx.x] = c;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) *count = part[0];
-}
-
-hipError_t launch_lad_count(const uint32_t* flags, int n, uint32_t gen, int* count, hipStream_t s) {
-  hipLaunchKernelGGL(lad_count_kernel, dim3(1), dim3(256), 0, s, flags, n, gen, count);
+hipError_t launch_lad_publish(const uint32_t* any, uint32_t gen, uint32_t* host, hipStream_t s) {
+  hipLaunchKernelGGL(lad_publish_kernel, dim3(1), dim3(64), 0, s, any, gen, host);
   return hipGetLastError();
 }
 
@@ -555,11 +550,11 @@ void lad_skip_bounds(const double* ramp, int nz, int& seg_first, int& src_imin, 
   str_imin = src_imin - TZ - LH - 1;
 }
 
-hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, const int* line_group, double thr, double ramp_min,
-                             hipStream_t s) {
+hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, uint32_t* any, const int* line_group, double thr,
+                             double ramp_min, hipStream_t s) {
   const long long n = (long long)a.g.nz * a.g.nr;
-  hipLaunchKernelGGL(lad_sensor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, flags, line_group,
-                     zfilter_segment(a.g.nz), thr, ramp_min);
+  hipLaunchKernelGGL(lad_sensor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, flags, any,
+                     line_group, zfilter_segment(a.g.nz), thr, ramp_min);
   return hipGetLastError();
 }
 
