@@ -202,9 +202,9 @@ typedef struct ajet_mlp {
  * ramp > ramp_min gets the reference's coefficients at each of the step's four stages instead:
  * the LAD kernels run only for those blocks and the tiles that feed them, and not at all in a
  * step that flags none.  The sensor of a state is taken after the step that produced it (same
- * state, same flags), so the host knows the count before the next step's first launch: each
- * step then waits once for the previous one.  sensor_threshold < 0 reproduces the reference's
- * LAD exactly. */
+ * state, same flags) and publishes whether it flagged a block to pinned host memory, so the host
+ * knows before the next step's first launch: each step then waits once (polling) for the
+ * previous one.  sensor_threshold < 0 reproduces the reference's LAD exactly. */
 int ajet_set_learned_lad(ajet_ctx* ctx, const ajet_mlp* model, double sensor_threshold, double ramp_min,
                          int32_t refresh);
 
