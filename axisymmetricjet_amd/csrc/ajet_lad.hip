@@ -202,7 +202,9 @@ __device__ __forceinline__ void source_tile(const LadArgs& a, int t, SourceLds& 
 }
 
 // Interior and boundary tiles in one launch, as the strain kernel.
-__global__ __launch_bounds__(256) void lad_source_kernel(LadArgs a) {
+// Seven blocks per CU: 22 KB of LDS allows them, and the bound holds the kernel to 72 VGPRs
+// (78 unbounded, which allows six), without spills.
+__global__ __launch_bounds__(256, 7) void lad_source_kernel(LadArgs a) {
   __shared__ SourceLds sh;
   for (int li = xcd_tile((int)blockIdx.x, (int)gridDim.x); li < a.ntiles; li += (int)gridDim.x) {
     const int e = a.tiles[li];
