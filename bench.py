@@ -378,7 +378,7 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
                      "traffic": MEASURED_TRAFFIC[args.method] * npts if MEASURED_TRAFFIC[args.method] and (nz, nr) == (4400, 640) else None,
-                     "kernel": "one RK stage: artificial-diffusivity launches (lad) + stage_kernel<stage>",
+                     "kernel": "the four RK stages of a step, each the artificial-diffusivity launches (lad) + the stage kernel",
                      "stage_ms": stage_ms, "bytes_per_update": BYTES_PER_UPDATE},
         "other_methods": others,
         "learned": learned,
