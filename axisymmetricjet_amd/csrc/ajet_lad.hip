@@ -149,28 +149,49 @@ __device__ __forceinline__ void source_tile(const LadArgs& a, int t, SourceLds& 
     }
   }
   const int tz = threadIdx.x % TZ;
+  // The window element k = threadIdx.x + 256 u of this thread sits at (wz, wr) = (k mod LWZ,
+  // k / LWZ), stepped from u = 0 by the constants 256 mod LWZ and 256 / LWZ.  The arrays are
+  // addressed flat: element k of sB, sZ, cZ, sR, cR is its k-th double or byte.
+  double* const fB = &sB[0][0];
+  double* const fZ = &sZ[0][0];
+  double* const fR = &sR[0][0];
+  const uint8_t* const fcZ = &cZ[0][0];
+  const uint8_t* const fcR = &cR[0][0];
+  const int wz0 = (int)threadIdx.x % LWZ, wr0 = (int)threadIdx.x / LWZ;
   double dz4[3][NP], dr4[3][NP], Tc[NP];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
     if (f > 0) __syncthreads();  // the previous field's stencils are read
     const double* __restrict__ src = f == 2 ? a.q + 4 * fs : a.A + f * fs;
-    for (int k = threadIdx.x; k < LWZ * LWR; k += blockDim.x) {
-      const int wz = k % LWZ, wr = k / LWZ;
+    int wz = wz0, wr = wr0;
+#pragma unroll 1
+    for (int k = threadIdx.x; k < LWZ * LWR; k += 256) {
       const int i = i0 - LH + wz, j = j0 - LH + wr;
-      const bool in = i >= 0 && i < nz && j >= 0 && j < nr;
+      const bool in = (unsigned)i < (unsigned)nz && (unsigned)j < (unsigned)nr;
       const double v = src[in ? (long long)i + (long long)pitch * j : 0];
-      sB[wr][wz] = in ? v : 0.0;
+      fB[k] = in ? v : 0.0;
+      wz += 256 % LWZ;
+      wr += 256 / LWZ;
+      if (wz >= LWZ) {
+        wz -= LWZ;
+        ++wr;
+      }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < TR * (TZ + 4); k += blockDim.x) {
-      const int c = k % (TZ + 4), ty = k / (TZ + 4);
-      const uint32_t zr = INT ? ROW_INTERIOR : cZ[ty][c];
-      sZ[ty][c] = d2_sum(zr, &sB[ty + LH][c - 2 + LH], 1);
+    // sZ[ty][c] and cZ[ty][c] are element k = 36 ty + c; its centre sB[ty + LH][c - 2 + LH]
+    // is element k + 10 ty + LH LWZ + LH - 2 of the window
+#pragma unroll 1
+    for (int k = threadIdx.x; k < TR * (TZ + 4); k += 256) {
+      const int ty = k / (TZ + 4);
+      const uint32_t zr = INT ? ROW_INTERIOR : fcZ[k];
+      fZ[k] = d2_sum(zr, fB + k + (LWZ - TZ - 4) * ty + LH * LWZ + LH - 2, 1);
     }
-    for (int k = threadIdx.x; k < (TR + 4) * TZ; k += blockDim.x) {
-      const int c = k % TZ, w = k / TZ;
-      const uint32_t rr = INT ? ROW_INTERIOR : cR[w][c];
-      sR[w][c] = d2_sum(rr, &sB[w - 2 + LH][c + LH], LWZ);
+    // sR[w][c] and cR[w][c] are element k = 32 w + c; its centre sB[w - 2 + LH][c + LH]
+#pragma unroll 1
+    for (int k = threadIdx.x; k < (TR + 4) * TZ; k += 256) {
+      const int w = k / TZ;
+      const uint32_t rr = INT ? ROW_INTERIOR : fcR[k];
+      fR[k] = d2_sum(rr, fB + k + (LWZ - TZ) * w + (LH - 2) * LWZ + LH, LWZ);
     }
     __syncthreads();
 #pragma unroll
