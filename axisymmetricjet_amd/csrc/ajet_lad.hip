@@ -426,13 +426,17 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
       const int l = j - jb;
       return (j >= 0 && j < nr && l >= 0 && l < ZLINES) ? ring[sl][l] : 0.0;
     };
+    // The sliding window of the interior rows reads its lines clamped to the group, without a
+    // test: an interior row's nine lines lie inside the grid and the group (the groups are built
+    // so), so every value it uses was read unclamped; the others use val().
+    auto win = [&](int j) { return ring[sl][min(max(j - jb, 0), ZLINES - 1)]; };
     double w[9];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) w[m + 1] = val(j0 - 4 + m);
+    for (int m = 0; m < 8; ++m) w[m + 1] = win(j0 - 4 + m);
     for (int j = j0; j < j1; ++j) {
 #pragma unroll
       for (int m = 0; m < 8; ++m) w[m] = w[m + 1];
-      w[8] = val(j + 4);
+      w[8] = win(j + 4);
       const int l = j - jb;
       const uint32_t rr = code[l][sl] >> 4;
       double v = 0.0;
