@@ -47,7 +47,7 @@ MLP_FLOP_PER_POINT = 2 * (104 * 128 + 2 * 128 * 128 + 128 * 2)
 # HBM bytes per grid-point update of a whole step, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE over the same workload (FETCH scaled by 2.0 and WRITE by 1.0, the factors a 1 GiB
 # device copy reads on the same counters); None where not measured.
-MEASURED_TRAFFIC = {"plain": 723.1, "lad": 1514.1, "learned": 914.4}  # profiles/step_jet4400x640_{plain,lad}.md, learned_lad_jet4400x640.md
+MEASURED_TRAFFIC = {"plain": 723.1, "lad": 1514.9, "learned": 890.1}  # profiles/step_jet4400x640_{plain,lad}.md, learned_lad_jet4400x640.md
 
 
 def log(msg):
@@ -87,7 +87,7 @@ def main(argv=None):
                     help="the ensemble's runs diverge after several hundred steps under the reference's "
                          "numerics, so 400 is the default rather than 2000")
     ap.add_argument("--learned-steps", type=int, default=1000, help="steps of the learned-LAD mode timed (0: skip)")
-    ap.add_argument("--learned-refresh", type=int, default=4, help="the network runs every this many steps")
+    ap.add_argument("--learned-refresh", type=int, default=8, help="the network runs every this many steps")
     ap.add_argument("--sensor-threshold", type=float, default=0.1)
     ap.add_argument("--ramp-min", type=float, default=1e-8)
     ap.add_argument("--score-steps", type=int, default=5000, help="steps of the learned-LAD scoring runs (0: skip)")

@@ -35,7 +35,7 @@ def test_trained_coefficients_beat_untrained_and_zero(case):
 
 
 def test_learned_mode_acoustics_beat_the_method_without_the_network(case):
-    res = ll.score(case, ll.load(), sensor_threshold=0.1, steps=1200, every=4, refresh=4)
+    res = ll.score(case, ll.load(), sensor_threshold=0.1, steps=1200, every=4, refresh=8)
     assert res["reference_finite"]
     lr, zr = res["learned"], res["zero"]
     print(f"OASPL error vs reference LAD, 1200 steps: learned max {lr['oaspl_err_db_max']:.3f} dB "
