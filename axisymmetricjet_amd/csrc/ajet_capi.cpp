@@ -530,12 +530,12 @@ static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
   const int mode = physics_mode(c->p);
   hipError_t e;
   if (mode == MODE_LAD && (e = launch_coefficients(c, stage, a.qin, c->stream)) != hipSuccess) return e;
-  if (mode == MODE_LAD && stage >= 1 && stage <= 3) {
+  if (stage_mixed_supported(stage, mode)) {
     // one launch for both tile classes: an event between two streams costs ~6 us of idle GPU
     // each way, and the stage kernel holds 2 waves per SIMD here for either class
     a.tiles = c->mix_tiles;
     a.ntiles = c->n_mix;
-    return launch_stage_mixed(stage, a, c->stream);
+    return launch_stage_mixed(stage, mode, a, c->stream);
   }
   if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;

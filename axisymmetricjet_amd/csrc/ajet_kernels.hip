@@ -538,8 +538,12 @@ hipError_t launch_stage(int stage, int mode, bool interior, const StageArgs& a, 
 }
 
 template <bool FORCE>
-static void launch_mixed_t(int stage, dim3 grid, const StageArgs& a, hipStream_t s) {
+static void launch_mixed_t(int stage, int mode, dim3 grid, const StageArgs& a, hipStream_t s) {
   const dim3 block(STAGE_THREADS);
+  if (mode == MODE_PLAIN) {
+    hipLaunchKernelGGL((stage_mixed_kernel<3, MODE_PLAIN, FORCE>), grid, block, 0, s, a);
+    return;
+  }
   switch (stage) {
     case 1: hipLaunchKernelGGL((stage_mixed_kernel<1, MODE_LAD, FORCE>), grid, block, 0, s, a); break;
     case 2: hipLaunchKernelGGL((stage_mixed_kernel<2, MODE_LAD, FORCE>), grid, block, 0, s, a); break;
@@ -547,11 +551,15 @@ static void launch_mixed_t(int stage, dim3 grid, const StageArgs& a, hipStream_t
   }
 }
 
-hipError_t launch_stage_mixed(int stage, const StageArgs& a, hipStream_t s) {
-  if (a.ntiles <= 0 || stage < 1 || stage > 3) return hipErrorInvalidValue;
+bool stage_mixed_supported(int stage, int mode) {
+  return (mode == MODE_LAD && stage >= 1 && stage <= 3) || (mode == MODE_PLAIN && stage == 3);
+}
+
+hipError_t launch_stage_mixed(int stage, int mode, const StageArgs& a, hipStream_t s) {
+  if (a.ntiles <= 0 || !stage_mixed_supported(stage, mode)) return hipErrorInvalidValue;
   const dim3 grid(pad8(a.ntiles));
-  if (a.forcing) launch_mixed_t<true>(stage, grid, a, s);
-  else launch_mixed_t<false>(stage, grid, a, s);
+  if (a.forcing) launch_mixed_t<true>(stage, mode, grid, a, s);
+  else launch_mixed_t<false>(stage, mode, grid, a, s);
   return hipGetLastError();
 }
 
