@@ -410,8 +410,16 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
   if (STAGE == 4) block_max_atomic(dmax, &a.sc->dmax_bits, red);
 }
 
+// Waves per SIMD the compiler must allow: the interior tiles of stages 0-2 without LAD and
+// forcing fit 3 (LDS 50.5 KB per block, at most 168 VGPRs, which the memory-clause scheduler
+// would otherwise exceed by a few); the others are held to 2 by LDS or registers.
 template <int STAGE, int MODE, bool INT, bool FORCE>
-__global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void stage_kernel(StageArgs a) {
+constexpr int stage_waves() {
+  return INT ? (MODE != MODE_LAD && !FORCE && STAGE <= 2 ? 3 : STAGE_WAVES_PER_EU) : 2;
+}
+
+template <int STAGE, int MODE, bool INT, bool FORCE>
+__global__ __launch_bounds__(STAGE_THREADS, (stage_waves<STAGE, MODE, INT, FORCE>())) void stage_kernel(StageArgs a) {
   AJET_STAGE_LDS(MODE);
   const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   if (li >= a.ntiles) return;
