@@ -41,6 +41,7 @@ struct ajet_ctx {
   int* lad_groups = nullptr;                       // line groups of the fused z/r filter
   int n_lad_groups = 0;
   int lad_seg_first = 0, lad_src_imin = -1, lad_str_imin = -1;  // zero-ramp work skipped
+  int lad_seg = 64;                                               // filter segment length
   StepScalars* sc = nullptr;
   double* dtlog = nullptr;
   long long logcap = 1 << 20;
@@ -235,7 +236,6 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   }
   std::vector<double> dzl, drl, ramp;
   lad_tables(c->s, nz, nr, dzl, drl, ramp);
-  lad_skip_bounds(ramp.data(), nz, c->lad_seg_first, c->lad_src_imin, c->lad_str_imin);
   struct V { double** d; const std::vector<double>* h; } vs[] = {
       {&c->zc, &c->s.zc}, {&c->rc, &c->s.rc}, {&c->dxidz, &c->s.dxidz}, {&c->ddxidxidz, &c->s.ddxidxidz},
       {&c->detadr, &c->s.detadr}, {&c->ddetadetadr, &c->s.ddetadetadr}, {&c->dzl, &dzl}, {&c->drl, &drl},
@@ -291,6 +291,9 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
     HIPCHK(c, hipMalloc((void**)&c->lad_groups, sizeof(int) * gr.size()));
     HIPCHK(c, hipMemcpy(c->lad_groups, gr.data(), sizeof(int) * gr.size(), hipMemcpyHostToDevice));
   }
+  c->lad_seg = lad_zfilter_segment(ramp.data(), nz, c->n_lad_groups, lad_zrfilter_slots(c->device));
+  c->nseg = (nz + c->lad_seg - 1) / c->lad_seg;
+  lad_skip_bounds(ramp.data(), nz, c->lad_seg, c->lad_seg_first, c->lad_src_imin, c->lad_str_imin);
   HIPCHK(c, hipMalloc((void**)&c->sc, sizeof(StepScalars)));
   HIPCHK(c, hipMemsetAsync(c->sc, 0, sizeof(StepScalars), c->stream));
   HIPCHK(c, hipMalloc((void**)&c->dtlog, sizeof(double) * (size_t)c->logcap));
@@ -412,6 +415,8 @@ static LadArgs lad_args(ajet_ctx* c, const double* q) {
   l.rgroups = c->lad_groups;
   l.nrgroups = c->n_lad_groups;
   l.seg_first = c->lad_seg_first;
+  l.zseg = c->lad_seg;
+  l.nseg = c->nseg;
   l.src_imin = c->lad_src_imin;
   l.str_imin = c->lad_str_imin;
   l.g = c->g;
@@ -739,7 +744,6 @@ int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_thres
     HIPCHK(c, hipMalloc((void**)&c->mlpB, sizeof(float) * (3 * MLP_H + MLP_NOUT)));
     HIPCHK(c, hipMalloc((void**)&c->mlpShift, sizeof(float) * MLP_NIN));
     HIPCHK(c, hipMalloc((void**)&c->mlpScale, sizeof(float) * MLP_NIN));
-    c->nseg = (g.nz + lad_zfilter_segment(g.nz) - 1) / lad_zfilter_segment(g.nz);
     HIPCHK(c, hipMalloc((void**)&c->zr_flags, sizeof(uint32_t) * (size_t)c->nseg * c->n_lad_groups));
     HIPCHK(c, hipMemset(c->zr_flags, 0, sizeof(uint32_t) * (size_t)c->nseg * c->n_lad_groups));
     HIPCHK(c, hipMalloc((void**)&c->any_dev, sizeof(uint32_t)));

@@ -89,7 +89,8 @@ struct LadArgs {
   const uint32_t* zr_flags;
   const uint8_t* src_mask;  // per tile: the sources are needed
   const uint8_t* str_mask;  // per tile: the strain rate and dilatation are needed
-  int nseg;
+  int nseg;                 // filter segments per line: ceil(nz / zseg)
+  int zseg;                 // filter segment length in columns, a multiple of 16, at least 64
   uint32_t gen;             // the step's stamp: a block is flagged when its flag equals it
   // where the ramp is exactly zero the coefficients are +-0 and only ever added, so that work is
   // skipped: filter segments below seg_first, source tiles ending at or before column src_imin,
@@ -101,8 +102,12 @@ hipError_t upload_tables(const double sbp1[9][6], const double sbp2[9][6], const
 hipError_t upload_lad_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
 // The three LAD launches (strain, source, fused z/r filter) on stream s.
 hipError_t launch_lad(const LadArgs& a, hipStream_t s);
-int lad_zfilter_segment(int nz);
-void lad_skip_bounds(const double* ramp, int nz, int& seg_first, int& src_imin, int& str_imin);
+// Resident blocks of the fused z/r filter on device (blocks per CU x CUs); 0 when unknown.
+int lad_zrfilter_slots(int device);
+// The filter segment length for a grid of nz columns, ramp[0..nz), ngroups line groups and
+// slots resident blocks (see ajet_lad.hip).
+int lad_zfilter_segment(const double* ramp, int nz, int ngroups, int slots);
+void lad_skip_bounds(const double* ramp, int nz, int seg, int& seg_first, int& src_imin, int& str_imin);
 // the dilatation sensor of the learned mode: flags[group(j) * nseg + i / seg] = gen where
 // max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min, and *any = gen when it flags a
 // block; then the publication of its result to host-visible memory, host[1] = (*any == gen) and

@@ -241,7 +241,7 @@ __global__ __launch_bounds__(256, 7) void lad_source_kernel(LadArgs a) {
 // before the centre), 0.467, so a line is cut into segments that each start ZWARM points early
 // from the unfiltered values: the start error is damped by 0.467^48 < 1.3e-16 before the first
 // point a segment writes.  A segment within ZWARM of the line start runs the exact recursion
-// from i = 1.
+// from i = 1.  The segment length is chosen per grid (lad_zfilter_segment).
 //
 // One wave walks 64 lines (one lane each) through its segment.  The columns stream through an
 // LDS ring of ZRING columns in chunks of ZCH: while the lanes filter chunk k, the loads of
@@ -257,10 +257,6 @@ constexpr int ZLINES = LAD_LINES;
 constexpr int ZCH = 16;
 constexpr int ZRING = 64;
 
-__host__ __device__ constexpr int zfilter_segment(int nz) {
-  // about 64 segments per line (several waves per CU), a multiple of the chunk, at least 64
-  return ((nz + 63) / 64 + ZCH - 1) / ZCH * ZCH < 64 ? 64 : ((nz + 63) / 64 + ZCH - 1) / ZCH * ZCH;
-}
 
 // Walk state of one lane: the last four filtered values and the next nine unfiltered ones.
 struct ZWalk {
@@ -304,7 +300,7 @@ __device__ __forceinline__ void zwalk_interior(ZWalk& st, double (*ring)[ZLINES 
 // ModShockCapturing.f90:64, 70, 76), scales it by C Re (C_kappa Re Pr) and the ramp
 // (ModShockCapturing.f90:66-92) and writes mu*, beta* or kappa*.  Blanked points are left out
 // by the filter and keep zero.  Ring codes: z row in the low nibble, r row in the high one.
-__global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg) {
+__global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a) {
   __shared__ double ring[ZRING][ZLINES + 1];
   __shared__ uint8_t code[ZLINES][ZRING + 4];  // [line][slot]
   const Geometry& g = a.g;
@@ -312,6 +308,7 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
   const long long fs = g.fstride;
   const int segi = (int)blockIdx.x + a.seg_first;  // segments before seg_first have zero ramps
   if (a.zr_flags && a.zr_flags[segi + a.nseg * (int)blockIdx.y] != a.gen) return;
+  const int seg = a.zseg;
   const int S0 = segi * seg;
   const int S1 = min(nz, S0 + seg);
   const int jb = a.rgroups[3 * blockIdx.y];
@@ -489,7 +486,7 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a, int seg
 // ---- learned mode: the dilatation sensor and the tile masks of the flagged filter blocks.
 __global__ __launch_bounds__(256) void lad_sensor_kernel(LadArgs a, uint32_t* __restrict__ flags,
                                                          uint32_t* __restrict__ any,
-                                                         const int* __restrict__ line_group, int seg, double thr,
+                                                         const int* __restrict__ line_group, double thr,
                                                          double ramp_min) {
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
@@ -507,7 +504,7 @@ __global__ __launch_bounds__(256) void lad_sensor_kernel(LadArgs a, uint32_t* __
   const double s = fmax(0.0, -dil) * fmax(a.dzl[i], a.drl[j]) / sqrt(at(4, i, j));
   if (s > thr) {
     // the step's stamp: every writer stores the same value, and older stamps need no clearing
-    uint32_t* f = &flags[line_group[j] * a.nseg + i / seg];
+    uint32_t* f = &flags[line_group[j] * a.nseg + i / a.zseg];
     if (*f != a.gen) *f = a.gen;
     if (*any != a.gen) *any = a.gen;
   }
@@ -524,9 +521,10 @@ __global__ void lad_publish_kernel(const uint32_t* __restrict__ any, uint32_t ge
 // A flagged block (segment sg, group gr) reads the sources over columns [ib - 16, S1 + 25)
 // (ib = S0 - warm-up) and lines [wb, wb + 64); a source tile reads the strain rate and
 // dilatation up to 7 points beyond itself, and a tile spans at most TZ x TR points.
-__global__ __launch_bounds__(256) void lad_mask_kernel(LadArgs a, const uint32_t* __restrict__ flags, int seg,
+__global__ __launch_bounds__(256) void lad_mask_kernel(LadArgs a, const uint32_t* __restrict__ flags,
                                                        uint8_t* __restrict__ src_mask, uint8_t* __restrict__ str_mask) {
   const Geometry& g = a.g;
+  const int seg = a.zseg;
   const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   if (t >= g.ntz * g.ntr) return;
   const int tzi = t % g.ntz, tri = t / g.ntz;
@@ -555,23 +553,55 @@ hipError_t launch_lad_publish(const uint32_t* any, uint32_t gen, uint32_t* host,
   return hipGetLastError();
 }
 
-int lad_zfilter_segment(int nz) { return zfilter_segment(nz); }
+int lad_zrfilter_slots(int device) {
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lad_zrfilter_kernel, ZLINES, 0) != hipSuccess)
+    return 0;
+  return cus * per_cu;
+}
 
-// The work the exactly-zero ramp makes unnecessary (the ramp rises with z).  A filter segment
-// whose columns all have ramp 0 writes only +-0; the first segment that does not starts its
-// walk ZWARM columns early and reads the sources from 16 columns before that; a source tile
-// reads the strain rate up to LH = 7 columns beyond itself and spans at most TZ columns.
-void lad_skip_bounds(const double* ramp, int nz, int& seg_first, int& src_imin, int& str_imin) {
+// Filter blocks from segment seg_first on (the ramp is zero before it).
+static int zfilter_first_segment(const double* ramp, int nz, int seg) {
   int ifirst = nz;
   for (int i = 0; i < nz; ++i)
     if (ramp[i] != 0.0) {
       ifirst = i;
       break;
     }
-  const int seg = zfilter_segment(nz);
   const int nseg = (nz + seg - 1) / seg;
-  seg_first = 0;
+  int seg_first = 0;
   while (seg_first < nseg && min(nz, (seg_first + 1) * seg) <= ifirst) ++seg_first;
+  return seg_first;
+}
+
+// The segment length: a multiple of ZCH and at least 64.  A block (one wave, held to a few per
+// CU by its LDS ring) walks its segment and ZWARM warm-up columns, and the blocks beyond the
+// slots that can be resident at once run in a further round; so the length that minimizes
+// rounds x (length + ZWARM) over the blocks the ramp leaves to run is taken, the shorter on a tie.
+// With slots <= 0 (unknown) every length takes one round and the shortest is taken.
+int lad_zfilter_segment(const double* ramp, int nz, int ngroups, int slots) {
+  int best = 64;
+  long long best_cost = -1;
+  const int top = max(64, (nz + ZCH - 1) / ZCH * ZCH);
+  for (int seg = 64; seg <= top; seg += ZCH) {
+    const long long blocks = 3ll * ngroups * ((nz + seg - 1) / seg - zfilter_first_segment(ramp, nz, seg));
+    const long long rounds = slots > 0 ? max(1ll, (blocks + slots - 1) / slots) : 1;
+    const long long cost = rounds * (min(seg, nz) + ZWARM);
+    if (best_cost < 0 || cost < best_cost) {
+      best = seg;
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+
+// The work the exactly-zero ramp makes unnecessary (the ramp rises with z).  A filter segment
+// whose columns all have ramp 0 writes only +-0; the first segment that does not starts its
+// walk ZWARM columns early and reads the sources from 16 columns before that; a source tile
+// reads the strain rate up to LH = 7 columns beyond itself and spans at most TZ columns.
+void lad_skip_bounds(const double* ramp, int nz, int seg, int& seg_first, int& src_imin, int& str_imin) {
+  seg_first = zfilter_first_segment(ramp, nz, seg);
   const int ib = max(0, seg_first * seg - ZWARM);
   src_imin = ib - ZCH;
   str_imin = src_imin - TZ - LH - 1;
@@ -581,15 +611,15 @@ hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, uint32_t* any, c
                              double ramp_min, hipStream_t s) {
   const long long n = (long long)a.g.nz * a.g.nr;
   hipLaunchKernelGGL(lad_sensor_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, flags, any,
-                     line_group, zfilter_segment(a.g.nz), thr, ramp_min);
+                     line_group, thr, ramp_min);
   return hipGetLastError();
 }
 
 hipError_t launch_lad_masks(const LadArgs& a, const uint32_t* flags, uint8_t* src_mask, uint8_t* str_mask,
                             hipStream_t s) {
   const int nt = a.g.ntz * a.g.ntr;
-  hipLaunchKernelGGL(lad_mask_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a, flags,
-                     zfilter_segment(a.g.nz), src_mask, str_mask);
+  hipLaunchKernelGGL(lad_mask_kernel, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, a, flags, src_mask,
+                     str_mask);
   return hipGetLastError();
 }
 
@@ -613,10 +643,9 @@ hipError_t launch_lad(const LadArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(lad_strain_kernel, grid(a.ntiles), dim3(256), 0, s, a);
     hipLaunchKernelGGL(lad_source_kernel, grid(a.ntiles), dim3(256), 0, s, a);
   }
-  const int seg = zfilter_segment(g.nz);
-  const int nseg = (g.nz + seg - 1) / seg;
+  const int nseg = (g.nz + a.zseg - 1) / a.zseg;
   if (nseg > a.seg_first)
-    hipLaunchKernelGGL(lad_zrfilter_kernel, dim3(nseg - a.seg_first, a.nrgroups, 3), dim3(ZLINES), 0, s, a, seg);
+    hipLaunchKernelGGL(lad_zrfilter_kernel, dim3(nseg - a.seg_first, a.nrgroups, 3), dim3(ZLINES), 0, s, a);
   return hipGetLastError();
 }
 

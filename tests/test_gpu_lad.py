@@ -52,8 +52,9 @@ def compressed(case, amp=0.08):
 CASES = {
     "jet440": lambda: jc.make_jet(440, 64, seed=1234),
     "swirl64": lambda: jc.make_jet(64, 40, seed=11, swirl=1e-3, r_max=16.0),
-    # lip strip at i = 130..133, inside the warm-up of the z-filter segment that starts at i = 145
-    "nozzle320": lambda: jc.make_nozzle(320, 96, i_lip=130, j_in=30, j_out=40, seed=4),
+    # lip closure rows at i = 150..153, inside the 48-point warm-up of the z-filter segment that
+    # starts at i = 192 (segments of 64 columns on a grid this small)
+    "nozzle320": lambda: jc.make_nozzle(320, 96, i_lip=150, j_in=30, j_out=40, seed=4),
     "walls200": lambda: jc.make_jet(200, 64, seed=5, bc_lines=gc.WALL_BC),
 }
 
