@@ -29,15 +29,28 @@ __device__ __forceinline__ void strain_tile(const LadArgs& a, int t, double (&sV
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
   if (i1 <= a.str_imin) return;  // only feeds zero-ramp columns
-  for (int k = threadIdx.x; k < WZ * WR; k += blockDim.x) {
+  // all of this thread's window loads are issued before the first store (one memory round trip)
+  constexpr int NW = (WZ * WR + 255) / 256;
+  double vr[NW], vz[NW];
+  bool vin[NW];
+#pragma unroll
+  for (int u = 0; u < NW; ++u) {
+    const int k = (int)threadIdx.x + 256 * u;
     const int wz = k % WZ, wr = k / WZ;
     const int i = i0 - HALO + wz, j = j0 - HALO + wr;
     // outside the grid: load point 0 (a valid address) and zero it, so no load is branched
-    const bool in = i >= 0 && i < nz && j >= 0 && j < nr;
-    const long long gi = in ? (long long)i + (long long)pitch * j : 0;
-    const double vr = a.q[gi + fs], vz = a.q[gi + 3 * fs];
-    sVr[wr][wz] = in ? vr : 0.0;
-    sVz[wr][wz] = in ? vz : 0.0;
+    vin[u] = k < WZ * WR && i >= 0 && i < nz && j >= 0 && j < nr;
+    const long long gi = vin[u] ? (long long)i + (long long)pitch * j : 0;
+    vr[u] = a.q[gi + fs];
+    vz[u] = a.q[gi + 3 * fs];
+  }
+#pragma unroll
+  for (int u = 0; u < NW; ++u) {
+    const int k = (int)threadIdx.x + 256 * u;
+    if (k < WZ * WR) {
+      (&sVr[0][0])[k] = vin[u] ? vr[u] : 0.0;
+      (&sVz[0][0])[k] = vin[u] ? vz[u] : 0.0;
+    }
   }
   __syncthreads();
   const int tz = threadIdx.x % TZ;
@@ -163,18 +176,33 @@ __device__ __forceinline__ void source_tile(const LadArgs& a, int t, SourceLds& 
   for (int f = 0; f < 3; ++f) {
     if (f > 0) __syncthreads();  // the previous field's stencils are read
     const double* __restrict__ src = f == 2 ? a.q + 4 * fs : a.A + f * fs;
+    // the thread's window elements in groups of SW whose loads are all issued before the first
+    // store, so it waits on a memory round trip per group instead of one per element (groups,
+    // not all at once: the kernel's 72 VGPRs hold three loads in flight without spilling)
+    constexpr int SW = 3;
+    constexpr int NG = ((LWZ * LWR + 255) / 256 + SW - 1) / SW;
     int wz = wz0, wr = wr0;
 #pragma unroll 1
-    for (int k = threadIdx.x; k < LWZ * LWR; k += 256) {
-      const int i = i0 - LH + wz, j = j0 - LH + wr;
-      const bool in = (unsigned)i < (unsigned)nz && (unsigned)j < (unsigned)nr;
-      const double v = src[in ? (long long)i + (long long)pitch * j : 0];
-      fB[k] = in ? v : 0.0;
-      wz += 256 % LWZ;
-      wr += 256 / LWZ;
-      if (wz >= LWZ) {
-        wz -= LWZ;
-        ++wr;
+    for (int gq = 0; gq < NG; ++gq) {
+      double v[SW];
+      bool vin[SW];
+#pragma unroll
+      for (int u = 0; u < SW; ++u) {
+        const int i = i0 - LH + wz, j = j0 - LH + wr;
+        vin[u] = (int)threadIdx.x + 256 * (SW * gq + u) < LWZ * LWR && (unsigned)i < (unsigned)nz &&
+                 (unsigned)j < (unsigned)nr;
+        v[u] = src[vin[u] ? (long long)i + (long long)pitch * j : 0];
+        wz += 256 % LWZ;
+        wr += 256 / LWZ;
+        if (wz >= LWZ) {
+          wz -= LWZ;
+          ++wr;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < SW; ++u) {
+        const int k = (int)threadIdx.x + 256 * (SW * gq + u);
+        if (k < LWZ * LWR) fB[k] = vin[u] ? v[u] : 0.0;
       }
     }
     __syncthreads();
