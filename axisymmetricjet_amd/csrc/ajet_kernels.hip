@@ -107,10 +107,11 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
   tile_range(tri, a.rcuts, j0, oj_lo, oj_hi);
 
   // ---- stage the window: primitives, p = rho T, mu = T**0.666 (AssignValues).  Every load
-  // of the thread's window points is issued before any of them is used, so the block waits
-  // for memory once.
+  // of the thread's window points (with LAD also mu* and kappa*) is issued before any of them
+  // is used, so the block waits for memory once.
   constexpr int NWIN = (WZ * WR + STAGE_THREADS - 1) / STAGE_THREADS;
   double win[NWIN][5];
+  double wlad[NWIN][2];  // mu*, kappa* (LAD)
   bool inside[NWIN];
 #pragma unroll
   for (int u = 0; u < NWIN; ++u) {
@@ -126,6 +127,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
       const double x = a.qin[gi + f * fs];
       win[u][f] = inside[u] ? x : 0.0;
     }
+    if constexpr (LAD) {
+      wlad[u][0] = a.lad[gi];
+      wlad[u][1] = a.lad[gi + 2 * fs];
+    }
   }
 #pragma unroll
   for (int u = 0; u < NWIN; ++u) {
@@ -138,10 +143,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
       mu = pow(win[u][4], 0.666);
       mphys = mu;
       kap = mu;
-      if (LAD) {
-        const long long gi = (long long)(i0 - HALO + wz) + (long long)pitch * (j0 - HALO + wr);
-        mu = mu + a.lad[gi];
-        kap = kap + a.lad[gi + 2 * fs];
+      if constexpr (LAD) {
+        mu = mu + wlad[u][0];
+        kap = kap + wlad[u][1];
       }
     }
 #pragma unroll
