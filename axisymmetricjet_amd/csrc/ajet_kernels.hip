@@ -113,6 +113,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
   double win[NWIN][5];
   double wlad[NWIN][2];  // mu*, kappa* (LAD)
   bool inside[NWIN];
+  // the xi metric of this thread's column in the pass below (the column is threadIdx.x mod TZ
+  // in every iteration), loaded with the window
+  static_assert(STAGE_THREADS % TZ == 0, "a thread keeps its column across the window rows");
+  const double mcol = a.dxidz[min(i0 + (int)threadIdx.x % TZ, nz - 1)];
 #pragma unroll
   for (int u = 0; u < NWIN; ++u) {
     const int k = threadIdx.x + u * STAGE_THREADS;
@@ -167,9 +171,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     double dvr = 0.0, dvz = 0.0;
     if (i >= oi_lo && i < oi_hi && j >= 0 && j < nr) {
       const uint32_t zr = INT ? ROW_INTERIOR : (a.meta[(long long)i + (long long)pitch * j] & 15u);
-      const double m = a.dxidz[i];
-      dvr = d1_sum(zr, &sF[1][wr][tz + HALO], 1) * m;
-      dvz = d1_sum(zr, &sF[3][wr][tz + HALO], 1) * m;
+      dvr = d1_sum(zr, &sF[1][wr][tz + HALO], 1) * mcol;
+      dvz = d1_sum(zr, &sF[3][wr][tz + HALO], 1) * mcol;
     }
     sD[0][wr][tz] = dvr;
     sD[1][wr][tz] = dvz;

@@ -29,7 +29,18 @@ __device__ __forceinline__ void strain_tile(const LadArgs& a, int t, double (&sV
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
   if (i1 <= a.str_imin) return;  // only feeds zero-ramp columns
-  // all of this thread's window loads are issued before the first store (one memory round trip)
+  // all of this thread's window loads, and the metrics of its points, are issued before the
+  // first store (one memory round trip)
+  constexpr int NP = (TR * TZ + 255) / 256;  // owned points per thread, rows 256 / TZ apart
+  static_assert(256 % TZ == 0, "a thread keeps its column across the rows");
+  const double mz = a.dxidz[min(i0 + (int)threadIdx.x % TZ, nz - 1)];
+  double mr[NP], rcj[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int j = min(j0 + (int)threadIdx.x / TZ + p * (256 / TZ), nr - 1);
+    mr[p] = a.detadr[j];
+    rcj[p] = a.rc[j];
+  }
   constexpr int NW = (WZ * WR + 255) / 256;
   double vr[NW], vz[NW];
   bool vin[NW];
@@ -56,18 +67,19 @@ __device__ __forceinline__ void strain_tile(const LadArgs& a, int t, double (&sV
   const int tz = threadIdx.x % TZ;
   const double c13 = 1.0 / 3.0;
   const double eps = (double)1e-15f;  // a default-REAL literal (ModShockCapturing.f90:202)
-  for (int ty = threadIdx.x / TZ; ty < TR; ty += blockDim.x / TZ) {
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int ty = (int)threadIdx.x / TZ + p * (256 / TZ);
     const int i = i0 + tz, j = j0 + ty;
-    if (i >= i1 || j >= j1) continue;
+    if (ty >= TR || i >= i1 || j >= j1) continue;
     const long long gi = (long long)i + (long long)pitch * j;
     const uint32_t meta = INT ? (ROW_INTERIOR | (ROW_INTERIOR << 4)) : a.meta[gi];
     const uint32_t zr = meta & 15u, rr = (meta >> 4) & 15u;
     const int wz = tz + HALO, wr = ty + HALO;
-    const double mz = a.dxidz[i], mr = a.detadr[j];
     const double dVrdz = d1_sum(zr, &sVr[wr][wz], 1) * mz;
     const double dVzdz = d1_sum(zr, &sVz[wr][wz], 1) * mz;
-    const double dVrdr = d1_sum(rr, &sVr[wr][wz], WZ) * mr;
-    const double dVzdr = d1_sum(rr, &sVz[wr][wz], WZ) * mr;
+    const double dVrdr = d1_sum(rr, &sVr[wr][wz], WZ) * mr[p];
+    const double dVzdr = d1_sum(rr, &sVz[wr][wz], WZ) * mr[p];
     const double Vr = sVr[wr][wz];
     // the axis row takes Vr/r -> dVr/dr (ModShockCapturing.f90:116-120, 146-150)
     double dil, S22;
@@ -75,7 +87,7 @@ __device__ __forceinline__ void strain_tile(const LadArgs& a, int t, double (&sV
       dil = 2.0 * dVrdr + dVzdz;
       S22 = dVrdr - c13 * dil;
     } else {
-      const double vbr = Vr / a.rc[j];
+      const double vbr = Vr / rcj[p];
       dil = (dVrdr + vbr) + dVzdz;
       S22 = vbr - c13 * dil;
     }
