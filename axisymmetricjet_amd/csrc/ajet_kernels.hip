@@ -116,7 +116,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
   // the xi metric of this thread's column in the pass below (the column is threadIdx.x mod TZ
   // in every iteration), loaded with the window
   static_assert(STAGE_THREADS % TZ == 0, "a thread keeps its column across the window rows");
-  const double mcol = a.dxidz[min(i0 + (int)threadIdx.x % TZ, nz - 1)];
+  const int icol = min(i0 + (int)threadIdx.x % TZ, nz - 1);
+  const double mcol = a.dxidz[icol], mcol2 = a.ddxidxidz[icol], zcol = a.zc[icol];
 #pragma unroll
   for (int u = 0; u < NWIN; ++u) {
     const int k = threadIdx.x + u * STAGE_THREADS;
@@ -220,10 +221,10 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     const long long gi = (long long)i + (long long)pitch * j;
     const uint32_t meta = INT ? ROW_INTERIOR | (ROW_INTERIOR << 4) : a.meta[gi];
     const uint32_t zr = INT ? ROW_INTERIOR : meta & 15u, rr = INT ? ROW_INTERIOR : (meta >> 4) & 15u;
-    const double mz = a.dxidz[i], mz2 = a.ddxidxidz[i];
+    const double mz = mcol, mz2 = mcol2;  // the column's metrics, loaded with the window
     const double mr = a.detadr[j], mr2 = a.ddetadetadr[j];
     const double r = a.rc[j];
-    const double zc = a.zc[i];
+    const double zc = zcol;
     // pointwise inputs of the stage update, loaded before the stencil work so that their
     // latency hides behind it: stage 2, 3: q^n (and S); stage 4: A and S
     double pin0[5], pin1[5];
