@@ -47,7 +47,7 @@ MLP_FLOP_PER_POINT = 2 * (104 * 128 + 2 * 128 * 128 + 128 * 2)
 # HBM bytes per grid-point update of a whole step, from rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE over the same workload (FETCH scaled by 2.0 and WRITE by 1.0, the factors a 1 GiB
 # device copy reads on the same counters); None where not measured.
-MEASURED_TRAFFIC = {"plain": 723.1, "lad": 1463.0, "learned": 891.0}  # profiles/step_jet4400x640_{plain,lad}.md, learned_lad_jet4400x640.md
+MEASURED_TRAFFIC = {"plain": 723.9, "lad": 1460.8, "learned": 891.0}  # profiles/step_jet4400x640_{plain,lad}.md, learned_lad_jet4400x640.md
 
 
 def log(msg):
