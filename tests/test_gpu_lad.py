@@ -56,6 +56,9 @@ CASES = {
     # starts at i = 192 (segments of 64 columns on a grid this small)
     "nozzle320": lambda: jc.make_nozzle(320, 96, i_lip=150, j_in=30, j_out=40, seed=4),
     "walls200": lambda: jc.make_jet(200, 64, seed=5, bc_lines=gc.WALL_BC),
+    # the full-size grid, whose z filter runs in 128-column segments on MI355X (1008 blocks, one
+    # per resident slot); the grids above take 64
+    "jet4400": lambda: jc.make_jet(4400, 640, seed=1234),
 }
 
 
