@@ -210,8 +210,9 @@ class Context:
             pass
 
     def set_state(self, q, target=None):
-        self._check(lib().ajet_set_state(self.h, F(q), None if target is None else F(target).ctypes.data),
-                    "ajet_set_state")
+        # the converted target stays referenced until the call returns
+        t = None if target is None else F(target)
+        self._check(lib().ajet_set_state(self.h, F(q), None if t is None else t.ctypes.data), "ajet_set_state")
 
     def state(self):
         out = np.zeros((self.nz, self.nr, 5), order="F")

@@ -44,7 +44,7 @@ struct ajet_ctx {
   int lad_seg = 64;                                               // filter segment length
   StepScalars* sc = nullptr;
   double* dtlog = nullptr;
-  long long logcap = 1 << 20;
+  long long logcap = AJET_DTLOG_CAP;
   bool dt_valid = false;     // sc->dt holds a time step
   bool fused_valid = false;  // sc->dmax_bits holds the CFL reduction of the current state
   bool pending_finalize = false;  // the next stage 1 turns that reduction into dt
@@ -54,6 +54,10 @@ struct ajet_ctx {
   ForcingDev fdev{};
   // learned LAD coefficient (ajet_set_learned_lad)
   bool learned = false;
+  double* lad_net = nullptr;               // the network's mu*, beta*, kappa* (3 fields), kept between refreshes
+  bool lad_restore = false;                // lad holds the reference's LAD in the blocks flagged at gen_written
+  bool lad_full_restore = false;           // lad was overwritten everywhere (test hooks): copy lad_net back
+  uint32_t gen_written = 0;                // stamp of the blocks the reference's LAD last wrote
   __bf16* mlpW = nullptr;                  // packed BF16 weights
   float *mlpB = nullptr, *mlpShift = nullptr, *mlpScale = nullptr;
   float out_scale[2] = {1.0f, 1.0f};
@@ -148,7 +152,7 @@ int ajet_destroy(ajet_ctx* c) {
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
                   c->int_tiles, c->bnd_tiles, c->mix_tiles, c->lad_tiles, c->lad_groups, c->codes, c->sc, c->dtlog,
                   c->mlpW, c->mlpB, c->mlpShift, c->mlpScale, c->zr_flags, c->src_mask, c->str_mask,
-                  c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out, c->any_dev};
+                  c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out, c->any_dev, c->lad_net};
   if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -168,6 +172,8 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   *out = nullptr;
   if (!p || !z || !r || (!lines && nlines > 0) || nlines < 0) return AJET_EINVAL;
   ajet_ctx* c = new ajet_ctx();
+  // the caller gets the context on every failure path below (for ajet_last_error) and destroys it
+  *out = c;
   c->p = *p;
   int rc = build_setup(*p, z, r, iblank, lines, nlines, c->s, c->err);
   if (rc == AJET_OK) rc = build_tiling(c->s, p->nz, p->nr, c->err);
@@ -184,7 +190,6 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   }
   if (rc != AJET_OK) {
     std::fprintf(stderr, "ajet_create: %s\n", c->err.c_str());
-    *out = c;  // so ajet_last_error can report; the caller destroys it
     return rc;
   }
   const int nz = p->nz, nr = p->nr;
@@ -302,7 +307,6 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   HIPCHK(c, upload_tables(A, B, F, c->stream));
   HIPCHK(c, upload_lad_tables(A, B, F, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  *out = c;
   return AJET_OK;
 }
 
@@ -447,7 +451,10 @@ static MlpInfer mlp_args(ajet_ctx* c, const double* q) {
   m.out_scale[0] = c->out_scale[0];
   m.out_scale[1] = c->out_scale[1];
   m.ramp_min = c->ramp_min;
-  m.out = c->lad;
+  // the network's coefficients are kept in lad_net; lad, which the stage kernels read, gets the
+  // same values and the reference's LAD in the blocks the sensor flags
+  m.out = c->lad_net;
+  m.out2 = c->lad;
   return m;
 }
 
@@ -502,15 +509,31 @@ static hipError_t learned_after_step(ajet_ctx* c, hipStream_t s) {
 
 static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStream_t s) {
   hipError_t e;
-  if (stage == 1 && c->learned_steps % c->refresh == 0 &&
-      (e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, s)) != hipSuccess)
-    return e;
+  if (stage == 1) {
+    if (c->learned_steps % c->refresh == 0) {
+      // the network: lad_net and lad
+      if ((e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, s)) != hipSuccess) return e;
+      c->lad_full_restore = c->lad_restore = false;
+    } else if (c->lad_full_restore) {
+      if ((e = hipMemcpyAsync(c->lad, c->lad_net, sizeof(double) * 3 * (size_t)c->g.fstride, hipMemcpyDeviceToDevice,
+                              s)) != hipSuccess)
+        return e;
+      c->lad_full_restore = c->lad_restore = false;
+    }
+  }
   if (stage == 4) ++c->learned_steps;
   bool any = false;
   if (stage == 1) {
     // the sensor of this state: taken after the previous step (wait for its result), or now
     if (!c->sensor_ready && (e = learned_sensor(c, q, s)) != hipSuccess) return e;
     c->sensor_ready = false;
+    if (c->lad_restore) {
+      // blocks the reference's LAD wrote in the previous step and the sensor no longer flags
+      // get the network's values back (blocks flagged again are rewritten below)
+      LadArgs l = lad_args(c, q);
+      if ((e = launch_lad_restore(l, c->zr_flags, c->gen_written, c->lad_net, c->lad, s)) != hipSuccess) return e;
+      c->lad_restore = false;
+    }
     if ((e = wait_sensor(c, &any)) != hipSuccess) return e;
     c->skip_lad = !any;
   }
@@ -523,6 +546,8 @@ static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStr
   l.zr_flags = c->zr_flags;
   l.src_mask = c->src_mask;
   l.str_mask = c->str_mask;
+  c->lad_restore = true;
+  c->gen_written = c->gen;
   return launch_lad(l, s);
 }
 
@@ -534,7 +559,14 @@ static hipError_t launch_coefficients(ajet_ctx* c, int stage, const double* q, h
 static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
   const int mode = physics_mode(c->p);
   hipError_t e;
-  if (mode == MODE_LAD && (e = launch_coefficients(c, stage, a.qin, c->stream)) != hipSuccess) return e;
+  if (mode == MODE_LAD && stage == 0) {
+    // the test hook's RHS takes the reference's coefficients of its input; in learned mode that
+    // overwrites lad, which the next step restores from the network's values in lad_net
+    if ((e = launch_lad(lad_args(c, a.qin), c->stream)) != hipSuccess) return e;
+    if (c->learned) c->lad_full_restore = true;
+  } else if (mode == MODE_LAD && (e = launch_coefficients(c, stage, a.qin, c->stream)) != hipSuccess) {
+    return e;
+  }
   if (stage_mixed_supported(stage, mode)) {
     // one launch for both tile classes: an event between two streams costs ~6 us of idle GPU
     // each way, and the stage kernel holds 2 waves per SIMD here for either class
@@ -681,9 +713,9 @@ int ajet_lad_coeffs(ajet_ctx* c, const double* q, double* mustar, double* betast
   if (c->p.shock_capturing != 1) return fail(c, AJET_EINVAL, "shock capturing is off in this context");
   HIPCHK(c, hipSetDevice(c->device));
   copy_state_in(c, c->Q2, q);
-  c->learned_steps = 0;  // the coefficient buffer is overwritten: the next step recomputes
-  c->sensor_ready = false;
   HIPCHK(c, launch_lad(lad_args(c, c->Q2), c->stream));
+  // in learned mode the next step restores lad from the network's values in lad_net
+  if (c->learned) c->lad_full_restore = true;
   double* outs[3] = {mustar, betastar, kappastar};
   for (int f = 0; f < 3; ++f)
     HIPCHK(c, hipMemcpy2DAsync(outs[f], sizeof(double) * c->g.nz, c->lad + (size_t)f * c->g.fstride,
@@ -727,8 +759,9 @@ int ajet_probe(ajet_ctx* c, double* values) {
 int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_threshold, double ramp_min,
                          int32_t refresh) {
   if (!c || !c->stream) return AJET_EINVAL;
-  c->learned_steps = 0;
+  c->learned_steps = 0;  // the next step runs the network, which writes lad_net and lad everywhere
   c->sensor_ready = false;
+  c->lad_restore = c->lad_full_restore = false;
   if (!model) {
     c->learned = false;
     return AJET_OK;
@@ -755,6 +788,10 @@ int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_thres
     HIPCHK(c, hipEventCreateWithFlags(&c->ev_sensor, hipEventDisableTiming));
     HIPCHK(c, hipMalloc((void**)&c->src_mask, (size_t)g.ntz * g.ntr));
     HIPCHK(c, hipMalloc((void**)&c->str_mask, (size_t)g.ntz * g.ntr));
+    HIPCHK(c, hipMemset(c->src_mask, 0, (size_t)g.ntz * g.ntr));
+    HIPCHK(c, hipMemset(c->str_mask, 0, (size_t)g.ntz * g.ntr));
+    HIPCHK(c, hipMalloc((void**)&c->lad_net, sizeof(double) * 3 * (size_t)g.fstride));
+    HIPCHK(c, hipMemset(c->lad_net, 0, sizeof(double) * 3 * (size_t)g.fstride));
     std::vector<int> lg((size_t)g.nr, 0);
     for (int k = 0; k < c->n_lad_groups; ++k)
       for (int j = c->lad_groups_h[3 * k + 1]; j < c->lad_groups_h[3 * k + 2]; ++j) lg[(size_t)j] = k;
@@ -809,12 +846,14 @@ int ajet_lad_learned_coeffs(ajet_ctx* c, const double* q, double* mustar, double
   if (!c->learned) return fail(c, AJET_EINVAL, "no learned model set");
   HIPCHK(c, hipSetDevice(c->device));
   copy_state_in(c, c->Q2, q);
-  c->learned_steps = 0;
-  c->sensor_ready = false;
-  HIPCHK(c, launch_mlp_infer(mlp_args(c, c->Q2), c->infer_blocks, c->stream));
+  // into the LAD kernels' scratch set, so that the context's coefficients stay as they are
+  MlpInfer m = mlp_args(c, c->Q2);
+  m.out = c->ladA;
+  m.out2 = nullptr;
+  HIPCHK(c, launch_mlp_infer(m, c->infer_blocks, c->stream));
   double* outs[3] = {mustar, betastar, kappastar};
   for (int f = 0; f < 3; ++f)
-    HIPCHK(c, hipMemcpy2DAsync(outs[f], sizeof(double) * c->g.nz, c->lad + (size_t)f * c->g.fstride,
+    HIPCHK(c, hipMemcpy2DAsync(outs[f], sizeof(double) * c->g.nz, c->ladA + (size_t)f * c->g.fstride,
                                sizeof(double) * c->g.pitch, sizeof(double) * c->g.nz, (size_t)c->g.nr,
                                hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

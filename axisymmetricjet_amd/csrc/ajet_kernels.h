@@ -117,6 +117,9 @@ hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, uint32_t* any, c
 hipError_t launch_lad_publish(const uint32_t* any, uint32_t gen, uint32_t* host, hipStream_t s);
 hipError_t launch_lad_masks(const LadArgs& a, const uint32_t* flags, uint8_t* src_mask, uint8_t* str_mask,
                             hipStream_t s);
+// dst = src (3 fields) over the output points of every z/r filter block whose flag equals gen
+hipError_t launch_lad_restore(const LadArgs& a, const uint32_t* flags, uint32_t gen, const double* src, double* dst,
+                              hipStream_t s);
 
 // ---- learned LAD coefficient (ajet_mlp.hip): 104 features -> 128 -> 128 -> 128 -> 2.
 constexpr int MLP_NIN = 104;       // features
@@ -175,6 +178,7 @@ struct MlpInfer {
   float out_scale[2];
   double ramp_min;        // groups whose ramp stays below this get zero coefficients
   double* out;            // 3 fields: mu* (zero), beta*, kappa*
+  double* out2;           // a second copy of out (may be null)
 };
 
 hipError_t launch_mlp_gemm(int epi, const MlpGemm& g, hipStream_t s);

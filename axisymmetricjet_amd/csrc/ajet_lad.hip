@@ -588,6 +588,32 @@ __global__ __launch_bounds__(256) void lad_mask_kernel(LadArgs a, const uint32_t
   str_mask[t] = str ? 1 : 0;
 }
 
+// Learned mode: the blocks the reference's LAD wrote at stamp gen get the network's values
+// back.  A z/r filter block (segment sg, group gr) writes columns [sg seg, (sg + 1) seg) of the
+// group's output lines and nothing else.
+__global__ __launch_bounds__(256) void lad_restore_kernel(LadArgs a, const uint32_t* __restrict__ flags, uint32_t gen,
+                                                          const double* __restrict__ src, double* __restrict__ dst) {
+  const int sg = (int)blockIdx.x, gr = (int)blockIdx.y;
+  if (flags[sg + a.nseg * gr] != gen) return;
+  const Geometry& g = a.g;
+  const int S0 = sg * a.zseg, S1 = min(g.nz, S0 + a.zseg);
+  const int lo = a.rgroups[3 * gr + 1], hi = a.rgroups[3 * gr + 2];
+  const int w = S1 - S0, n = w * (hi - lo);
+  for (int f = 0; f < 3; ++f)
+    for (int k = (int)threadIdx.x; k < n; k += (int)blockDim.x) {
+      const long long gi = (long long)f * g.fstride + (S0 + k % w) + (long long)g.pitch * (lo + k / w);
+      dst[gi] = src[gi];
+    }
+}
+
+hipError_t launch_lad_restore(const LadArgs& a, const uint32_t* flags, uint32_t gen, const double* src, double* dst,
+                              hipStream_t s) {
+  if (a.nseg <= 0 || a.nrgroups <= 0) return hipSuccess;
+  hipLaunchKernelGGL(lad_restore_kernel, dim3((unsigned)a.nseg, (unsigned)a.nrgroups), dim3(256), 0, s, a, flags, gen,
+                     src, dst);
+  return hipGetLastError();
+}
+
 hipError_t launch_lad_publish(const uint32_t* any, uint32_t gen, uint32_t* host, hipStream_t s) {
   hipLaunchKernelGGL(lad_publish_kernel, dim3(1), dim3(64), 0, s, any, gen, host);
   return hipGetLastError();

@@ -279,7 +279,10 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void mlp_infer_kernel(MlpInfer a)
       // the ramp (increasing in z) is below the cut-off over the whole group: no LAD here
       for (int r = q; r < 16; r += 4) {
         const int i = i0 + r;
-        if (p < 3 && i < nz) a.out[(long long)p * fs + i + (long long)pitch * j] = 0.0;
+        if (p < 3 && i < nz) {
+          a.out[(long long)p * fs + i + (long long)pitch * j] = 0.0;
+          if (a.out2) a.out2[(long long)p * fs + i + (long long)pitch * j] = 0.0;
+        }
       }
       continue;
     }
@@ -347,9 +350,16 @@ __global__ __launch_bounds__(64 * INFER_WAVES) void mlp_infer_kernel(MlpInfer a)
     if (q == 0 && i < nz) {
       const long long gi = (long long)i + (long long)pitch * j;
       const double r = a.ramp[i];
+      const double b = r * (double)a.out_scale[0] * (double)fmaxf(y[0] + sBias[3 * MLP_H], 0.0f);
+      const double k = r * (double)a.out_scale[1] * (double)fmaxf(y[1] + sBias[3 * MLP_H + 1], 0.0f);
       a.out[gi] = 0.0;
-      a.out[fs + gi] = r * (double)a.out_scale[0] * (double)fmaxf(y[0] + sBias[3 * MLP_H], 0.0f);
-      a.out[2 * fs + gi] = r * (double)a.out_scale[1] * (double)fmaxf(y[1] + sBias[3 * MLP_H + 1], 0.0f);
+      a.out[fs + gi] = b;
+      a.out[2 * fs + gi] = k;
+      if (a.out2) {
+        a.out2[gi] = 0.0;
+        a.out2[fs + gi] = b;
+        a.out2[2 * fs + gi] = k;
+      }
     }
     wave_order();
   }

@@ -5,6 +5,7 @@ MODULE ajet_capi
   IMPLICIT NONE
 
   INTEGER(C_INT), PARAMETER :: AJET_OK = 0, AJET_EBLOWUP = -5, AJET_ENONFINITE = -6
+  INTEGER(C_INT), PARAMETER :: AJET_DTLOG_CAP = 1048576  ! entries the device time-step log holds
 
   ! ajet_params (solver constants; ModuleVariables.f90:12-13, 66, 71, 73, 152, 164, 177)
   TYPE, BIND(C) :: ajet_params_t

@@ -178,6 +178,10 @@ CONTAINS
     REAL(REAL64) :: tt
     n = ajet_get_dt_log(ctx, C_NULL_PTR, 0_C_INT32_T, C_LOC(t))
     IF (n < 0) CALL check(n, 'ajet_get_dt_log')
+    ! the device log keeps the first AJET_DTLOG_CAP time steps
+    IF (n > AJET_DTLOG_CAP .AND. nlogged < AJET_DTLOG_CAP) &
+      PRINT *, 'time-step log full: no Iteration lines past ', iteration_start + AJET_DTLOG_CAP
+    n = MIN(n, AJET_DTLOG_CAP)
     IF (n <= nlogged) RETURN
     ALLOCATE (dts(n))
     n = ajet_get_dt_log(ctx, C_LOC(dts), INT(n, C_INT32_T), C_NULL_PTR)

@@ -139,11 +139,14 @@ int ajet_rk4_stage(ajet_ctx* ctx, int32_t stage);
 int ajet_run(ajet_ctx* ctx, int32_t nsteps);
 
 /* Time steps and accumulated time since ajet_set_state (ModRungeKutta.f90:152-155).
- * Copies min(n, logged) entries to dts (may be NULL) and returns the number logged. */
+ * Copies min(n, logged, AJET_DTLOG_CAP) entries to dts (may be NULL) and returns the number of
+ * steps logged; the device log holds the first AJET_DTLOG_CAP of them (the time keeps adding up). */
+enum { AJET_DTLOG_CAP = 1 << 20 };
 int ajet_get_dt_log(ajet_ctx* ctx, double* dts, int32_t n, double* time);
 
 /* Test hook: one ComputeRHS (ModRHS.f90:5-258) of the state q (nz, nr, 5), with AssignValues
- * (ModAssign.f90:5-29) before it; writes rhs (nz, nr, 5).  Does not change the context state. */
+ * (ModAssign.f90:5-29) before it; writes rhs (nz, nr, 5).  With shock capturing it takes the
+ * reference's LAD coefficients of q, in the learned mode too.  Does not change the context state. */
 int ajet_rhs(ajet_ctx* ctx, const double* q, double* rhs);
 
 /* Test hook: the artificial transport coefficients mu*, beta*, kappa* that
@@ -199,7 +202,8 @@ typedef struct ajet_mlp {
  * (zero elsewhere); they are kept until the next refresh.  At the first stage of every step a
  * dilatation sensor s = max(0, -dil) max(dz, dr) / sqrt(T) is taken, and every z/r filter block
  * (a z segment times an r line group) holding a point with s > sensor_threshold and
- * ramp > ramp_min gets the reference's coefficients at each of the step's four stages instead:
+ * ramp > ramp_min gets the reference's coefficients at each of the step's four stages instead (and
+ * the network's values again in the next step whose sensor does not flag it):
  * the LAD kernels run only for those blocks and the tiles that feed them, and not at all in a
  * step that flags none.  The sensor of a state is taken after the step that produced it (same
  * state, same flags) and publishes whether it flagged a block to pinned host memory, so the host

@@ -209,3 +209,73 @@ def test_refresh_interval():
     assert not np.array_equal(runs[1][1], runs[3][1])
     with pytest.raises(capi.AjetError):
         capi.Context(case, **LAD).set_learned_lad(m, 1e30, 1e-8, refresh=0)
+
+
+def zero_model():
+    """A network whose output is zero everywhere (all weights and biases zero)."""
+    m = {k: np.zeros_like(v) for k, v in mo.init_model(0).items()}
+    m["scale"] = np.ones(mo.NIN, np.float32)
+    m["out_scale"] = np.ones(2, np.float32)
+    return m
+
+
+def compressed_jet():
+    case = jc.make_jet(440, 64, seed=5)
+    z = case.z[:, 0]
+    q = np.array(case.q0, order="F")
+    q[:, :, 3] += (-0.3 * np.tanh((z - 20.0) / 0.2))[:, None]
+    return case, q
+
+
+def test_unflagged_blocks_return_to_the_network_values():
+    """A block the sensor flagged takes the reference's coefficients, and the network's again in
+    the next step that does not flag it.  With a network that gives zero everywhere the network's
+    values never change, so running the network every step (refresh 1) and every third step
+    (refresh 3) must give the same run bit for bit -- also across steps where the sensor lets go
+    of blocks it flagged before (the flagged fraction falls)."""
+    case, q = compressed_jet()
+    m = zero_model()
+    found = False
+    for thr in (0.08, 0.06, 0.04, 0.03, 0.02):
+        c3 = capi.Context(case, **LAD)
+        c3.set_state(q, case.target)
+        c3.set_learned_lad(m, sensor_threshold=thr, ramp_min=1e-8, refresh=3)
+        states, fracs = [], []
+        for _ in range(12):
+            c3.run(1)
+            states.append(c3.state())
+            fracs.append(c3.fallback_fraction())
+        print(f"threshold {thr}: flagged fractions {['%.3f' % f for f in fracs]}", flush=True)
+        if not any(b < a for a, b in zip(fracs, fracs[1:])) or fracs[0] == 0.0:
+            continue
+        found = True
+        c1 = capi.Context(case, **LAD)
+        c1.set_state(q, case.target)
+        c1.set_learned_lad(m, sensor_threshold=thr, ramp_min=1e-8, refresh=1)
+        for k, s in enumerate(states):
+            c1.run(1)
+            assert np.array_equal(c1.state(), s), f"step {k + 1}"
+        break
+    assert found, "no threshold made the sensor let go of a block"
+
+
+def test_hooks_leave_a_learned_run_unchanged():
+    """ajet_rhs and ajet_lad_coeffs take the reference's coefficients of their input; on a
+    learned context the next step restores the network's values, so the run goes on exactly as
+    one where the hooks were never called."""
+    case, q = compressed_jet()
+    m = rand_model(15, shift_scale=True)
+    a = capi.Context(case, **LAD)
+    b = capi.Context(case, **LAD)
+    for c in (a, b):
+        c.set_state(q, case.target)
+        c.set_learned_lad(m, sensor_threshold=0.05, ramp_min=1e-8, refresh=3)
+        c.run(1)
+    a.rhs(case.q0)
+    a.lad_coeffs(q)
+    a.learned_coeffs(case.q0)
+    a.run(4)
+    b.run(4)
+    assert b.fallback_fraction() > 0.0
+    assert np.array_equal(a.dt_log()[0], b.dt_log()[0])
+    assert np.array_equal(a.state(), b.state())
