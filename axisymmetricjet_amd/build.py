@@ -13,7 +13,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libajet.so")
-SOURCES = ["ajet_kernels.hip", "ajet_lad.hip", "ajet_mlp.hip", "ajet_capi.cpp", "ajet_setup.cpp", "ajet_trainer.cpp"]
+SOURCES = ["ajet_kernels.hip", "ajet_lad.hip", "ajet_mlp.hip", "ajet_capi.cpp", "ajet_setup.cpp", "ajet_trainer.cpp",
+           "ajet_noalloc.cpp"]
 HEADERS = ["ajet_internal.h", "ajet_kernels.h", "ajet_physics.h", "ajet_stencil.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

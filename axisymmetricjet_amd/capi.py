@@ -23,7 +23,8 @@ EXPORTS = ["ajet_params_default", "ajet_build_patches", "ajet_create", "ajet_des
            "ajet_set_learned_lad", "ajet_lad_features", "ajet_lad_learned_coeffs", "ajet_lad_fallback_fraction",
            "ajet_trainer_create", "ajet_trainer_load", "ajet_trainer_run", "ajet_trainer_forward",
            "ajet_trainer_grads", "ajet_trainer_get", "ajet_trainer_destroy", "ajet_trainer_last_error",
-           "ajet_set_probes", "ajet_probe"]
+           "ajet_set_probes", "ajet_probe", "ajet_probe_at"]
+PROBE_AT_MAX = 4096  # AJET_PROBE_AT_MAX
 
 
 class Params(C.Structure):
@@ -97,6 +98,7 @@ def lib():
         L.ajet_trainer_last_error.restype = C.c_char_p
         L.ajet_set_probes.argtypes = [vp, vp, C.c_int32]
         L.ajet_probe.argtypes = [vp, vp]
+        L.ajet_probe_at.argtypes = [vp, vp, C.c_int32, vp]
         L.ajet_stream.argtypes = [vp]
         L.ajet_stream.restype = vp
         L.ajet_version.restype = C.c_char_p
@@ -298,6 +300,14 @@ class Context:
         """(n, 5) primitives at the probe points of the current state."""
         out = np.zeros((len(self._probe_pts), 5))
         self._check(lib().ajet_probe(self.h, out.ctypes.data), "ajet_probe")
+        return out
+
+    def probe_at(self, points):
+        """(n, 5) primitives at the points (n, 2) = (i, j), 0-based, of the current state, through
+        the library's fixed scratch area (ajet_probe_at: no device allocation, n <= PROBE_AT_MAX)."""
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.int32).reshape(-1, 2))
+        out = np.zeros((len(pts), 5))
+        self._check(lib().ajet_probe_at(self.h, pts.ctypes.data, len(pts), out.ctypes.data), "ajet_probe_at")
         return out
 
     def stream(self):

@@ -754,6 +754,23 @@ int ajet_probe(ajet_ctx* c, double* values) {
   return AJET_OK;
 }
 
+}  // extern "C"
+
+bool ajet::ctx_probe_view(ajet_ctx* c, ProbeView* v) {
+  if (!c || !c->stream) return false;
+  v->device = c->device;
+  v->stream = c->stream;
+  v->q = c->Q0;
+  v->g = c->g;
+  return true;
+}
+
+void ajet::ctx_set_error(ajet_ctx* c, const std::string& msg) {
+  if (c) c->err = msg;
+}
+
+extern "C" {
+
 // ---------------------------------------------------------------- learned LAD coefficient
 
 int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_threshold, double ramp_min,

@@ -209,4 +209,15 @@ hipError_t launch_cfl(const CflArgs& a, double CFL, int any_blank, double* log, 
 hipError_t launch_finalize(StepScalars* sc, double CFL, int any_blank, double* log, long long cap, hipStream_t s);
 hipError_t launch_probe(const double* q, const Geometry& g, const int* pts, int n, double* out, hipStream_t s);
 
+// What ajet_probe_at (ajet_noalloc.cpp) needs of a context: its device, stream, current state
+// and geometry.  False for a context without a stream.
+struct ProbeView {
+  int device;
+  hipStream_t stream;
+  const double* q;
+  Geometry g;
+};
+bool ctx_probe_view(ajet_ctx* c, ProbeView* v);
+void ctx_set_error(ajet_ctx* c, const std::string& msg);
+
 }  // namespace ajet

@@ -249,6 +249,17 @@ const char* ajet_trainer_last_error(const ajet_trainer* tr);
 int ajet_set_probes(ajet_ctx* ctx, const int32_t* points, int32_t n);
 int ajet_probe(ajet_ctx* ctx, double* values);
 
+/* Probes without device-memory allocation -- no counterpart in the reference; for callers that
+ * cannot allocate device memory (tight sampling loops, harnesses that forbid hipMalloc): the
+ * values ajet_set_probes(points, n) followed by ajet_probe would give, values[5 k + v] =
+ * pv(i_k, j_k, v) of the current state, with the n points (0-based (i, j) pairs) and the values
+ * passed through a fixed scratch area of AJET_PROBE_AT_MAX points that the library holds on each
+ * GPU for the life of the process.  n > AJET_PROBE_AT_MAX is AJET_EINVAL.  Every call on one GPU
+ * uses the same scratch area: calls that overlap on one GPU (contexts on other streams, other
+ * host threads) race.  Leaves the points of ajet_set_probes as they are.  Synchronises. */
+enum { AJET_PROBE_AT_MAX = 4096 };
+int ajet_probe_at(ajet_ctx* ctx, const int32_t* points, int32_t n, double* values);
+
 /* The HIP stream the context launches on (a hipStream_t), for callers that time it. */
 void* ajet_stream(ajet_ctx* ctx);
 
