@@ -20,8 +20,12 @@ HEADERS = ["ajet_internal.h", "ajet_kernels.h", "ajet_physics.h", "ajet_stencil.
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # -ffp-contract=off: every multiply and add rounds on its own, as in the reference's
 # unoptimised build; packed FP32 is disabled on the device side.
+# -fmacf64-inst: FP64 multiply-adds take the three-operand form, whose addend may sit in scalar
+# registers, rather than the accumulate form, which first moves a constant addend into a
+# vector register pair (two extra vector instructions per constant; same arithmetic).
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Xclang", "-target-feature", "-Xclang", "-fmacf64-inst",
          "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
 # Per-source additions.  The stage kernels are latency-bound at 2-3 waves per SIMD; the
 # scheduler that clusters their memory instructions measured 1.375 -> 1.341 ms per LAD step

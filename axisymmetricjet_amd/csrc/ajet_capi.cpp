@@ -3,6 +3,7 @@
 // temperature of the CFL rule and the accumulated time live on the device.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -28,6 +29,7 @@ struct ajet_ctx {
   double *dzl = nullptr, *drl = nullptr, *ramp = nullptr;   // LAD grid spacings and ramp (1-D)
   uint32_t* meta = nullptr;
   uint8_t* codes = nullptr;  // low byte of meta (the LAD filter's row codes)
+  double* rinv = nullptr;  // 1/r(j), 0 on the axis (interior-tile stage kernels)
   double *zc = nullptr, *rc = nullptr, *dxidz = nullptr, *ddxidxidz = nullptr, *detadr = nullptr, *ddetadetadr = nullptr;
   PatchDev* patches = nullptr;
   uint8_t* seq = nullptr;
@@ -152,7 +154,7 @@ int ajet_destroy(ajet_ctx* c) {
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
                   c->int_tiles, c->bnd_tiles, c->mix_tiles, c->lad_tiles, c->lad_groups, c->codes, c->sc, c->dtlog,
                   c->mlpW, c->mlpB, c->mlpShift, c->mlpScale, c->zr_flags, c->src_mask, c->str_mask,
-                  c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out, c->any_dev, c->lad_net};
+                  c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out, c->any_dev, c->lad_net, c->rinv};
   if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -212,6 +214,16 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   c->pc.sponge_end_left = p->sponge_end_left;
   c->pc.sponge_A_left = p->sponge_A_left;
   c->pc.r_top = c->s.rc[nr - 1];
+  c->pc.inv_gam = 1.0 / p->gam;
+  c->pc.gm1 = p->gam - 1.0;
+  c->pc.inv_Re = 1.0 / p->Re;
+  c->pc.g_RePr = p->gam / (p->Re * p->Pr);
+  c->pc.g_gm1_Re = p->gam * (p->gam - 1.0) / p->Re;
+  c->pc.cfl_visc = std::max(1.0 / p->Re, 1.0 / (p->Re * p->Pr));
+  if (5ull * (unsigned long long)g.fstride * sizeof(double) >= (1ull << 31)) {
+    std::fprintf(stderr, "ajet_create: grid too large\n");
+    return fail(c, AJET_EGRID, "a state of the grid needs 2 GiB or more (the kernels address it with 31-bit offsets)");
+  }
 
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
@@ -241,10 +253,12 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   }
   std::vector<double> dzl, drl, ramp;
   lad_tables(c->s, nz, nr, dzl, drl, ramp);
+  std::vector<double> rinv((size_t)nr);
+  for (int j = 0; j < nr; ++j) rinv[(size_t)j] = c->s.rc[(size_t)j] > 0.0 ? 1.0 / c->s.rc[(size_t)j] : 0.0;
   struct V { double** d; const std::vector<double>* h; } vs[] = {
       {&c->zc, &c->s.zc}, {&c->rc, &c->s.rc}, {&c->dxidz, &c->s.dxidz}, {&c->ddxidxidz, &c->s.ddxidxidz},
       {&c->detadr, &c->s.detadr}, {&c->ddetadetadr, &c->s.ddetadetadr}, {&c->dzl, &dzl}, {&c->drl, &drl},
-      {&c->ramp, &ramp}};
+      {&c->ramp, &ramp}, {&c->rinv, &rinv}};
   for (auto& v : vs) {
     HIPCHK(c, hipMalloc((void**)v.d, sizeof(double) * v.h->size()));
     HIPCHK(c, hipMemcpyAsync(*v.d, v.h->data(), sizeof(double) * v.h->size(), hipMemcpyHostToDevice, c->stream));
@@ -370,6 +384,7 @@ static StageArgs stage_args(ajet_ctx* c) {
   a.dxidz = c->dxidz;
   a.ddxidxidz = c->ddxidxidz;
   a.rc = c->rc;
+  a.rinv = c->rinv;
   a.detadr = c->detadr;
   a.ddetadetadr = c->ddetadetadr;
   a.lad = c->lad;

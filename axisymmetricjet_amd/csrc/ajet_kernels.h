@@ -28,6 +28,7 @@ struct StageArgs {
   const double* dxidz;     // (nz)
   const double* ddxidxidz;
   const double* rc;        // r(j)
+  const double* rinv;      // 1/r(j) (0 on the axis)
   const double* detadr;    // (nr)
   const double* ddetadetadr;
   const double* lad;       // mu*, beta*, kappa* (3 fields) when shock capturing is on

@@ -1,6 +1,7 @@
 // gfx950 kernels of the RK4 time step: the fused stage kernel (ComputeRHS + stage update +
 // axis fix, with the CFL reduction of the next step fused into the last stage), the
-// stand-alone CFL reduction, and its finalisation.  Build with -ffp-contract=off.
+// stand-alone CFL reduction, and its finalisation.  Build with -ffp-contract=off: the only fused
+// multiply-adds are the explicit fma() calls of the interior-tile forms.
 #include <hip/hip_runtime.h>
 
 #include "ajet_kernels.h"
@@ -71,10 +72,12 @@ __device__ __forceinline__ void block_max_atomic(double v, unsigned long long* d
 
 // NF staged fields: rho, Vr, Vt, Vz, T, p, mu, kappa, T**0.666 (the last two only with LAD:
 // without it kappa = mu = T**0.666, AssignValues ModAssign.f90:19-22).
-// INT: the tile list holds interior tiles only -- every owned point and every window row of
-// the owned columns takes the interior stencils, and no owned point is on the axis or on a
-// boundary patch -- so the metadata word, the closures, the patches and the axis fix are
-// compiled out.
+// INT: the tile list holds interior tiles only -- the window lies inside the grid, every owned
+// point and every window row of the owned columns takes the interior stencils, and no owned
+// point is on the axis or on a boundary patch -- so the metadata word, the closures, the
+// patches, the axis fix and the window's bounds tests are compiled out, and the arithmetic takes
+// the fused forms (ajet_fastmath.h, inviscid_fast, viscous_fast).  Boundary tiles keep the
+// reference's operation order.
 // MODE: MODE_PLAIN (viscous), MODE_SC (viscous with the per-point coefficients of
 // ShockCapturing = 1 and zero artificial terms), MODE_LAD (with mu*, beta*, kappa*),
 // MODE_INVISCID (do_viscous_terms = 0).
@@ -89,6 +92,30 @@ constexpr int stage_nf() { return MODE == MODE_LAD ? 9 : 7; }
   __shared__ double sD[2][WR][TZ];                            \
   __shared__ double red[STAGE_THREADS / 64]
 
+// Interior tiles address fields through raw buffer descriptors: one 32-bit byte offset per
+// point (voffset) shared by every field, the field's byte offset f * fstride * 8 in a scalar
+// register (soffset), so no access needs vector address arithmetic.  ajet_create keeps a state's
+// five fields below 2 GiB.  The descriptor's fourth word is the raw-buffer format of the gfx9
+// family (gfx950 included).
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc rsrc(const double* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(p), 0, 0x7fffffff, 0x00020000);
+}
+__device__ __forceinline__ double bld(Rsrc r, uint32_t voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void bst(Rsrc r, uint32_t voff, int soff, double v) {
+  using W = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(W, v), r, voff, soff, 0);
+}
+
+// Interior-tile first and second derivatives on an LDS line (stride s): the factored forms of
+// ajet_fastmath.h.
+__device__ __forceinline__ double d1i(const double* c, int s) { return d1_int(c[-2 * s], c[-s], c[s], c[2 * s]); }
+__device__ __forceinline__ double d2i(const double* c, int s) {
+  return d2_int(c[-2 * s], c[-s], c[0], c[s], c[2 * s]);
+}
+
 template <int STAGE, int MODE, bool INT, bool FORCE>
 __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool writer, int dt_from_dmax, int parity,
                                            double (&sF)[stage_nf<MODE>()][WR][WZ], double (&sD)[2][WR][TZ],
@@ -101,6 +128,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
+  const int fsb = (int)(8 * fs);  // a field's byte stride (interior tiles' buffer offsets)
   const int tzi = t % g.ntz, tri = t / g.ntz;
   int i0, oi_lo, oi_hi, j0, oj_lo, oj_hi;
   tile_range(tzi, a.zcuts, i0, oi_lo, oi_hi);
@@ -108,7 +136,8 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
 
   // ---- stage the window: primitives, p = rho T, mu = T**0.666 (AssignValues).  Every load
   // of the thread's window points (with LAD also mu* and kappa*) is issued before any of them
-  // is used, so the block waits for memory once.
+  // is used, so the block waits for memory once.  An interior tile's window lies inside the
+  // grid (classify_tiles), so it loads without bounds tests.
   constexpr int NWIN = (WZ * WR + STAGE_THREADS - 1) / STAGE_THREADS;
   double win[NWIN][5];
   double wlad[NWIN][2];  // mu*, kappa* (LAD)
@@ -123,18 +152,32 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     const int k = threadIdx.x + u * STAGE_THREADS;
     const int wz = k % WZ, wr = k / WZ;
     const int i = i0 - HALO + wz, j = j0 - HALO + wr;
-    inside[u] = k < WZ * WR && i >= 0 && i < nz && j >= 0 && j < nr;
-    // points outside the grid load point 0 (a valid address) and are zeroed after the load,
-    // so every load issues without a branch
-    const long long gi = inside[u] ? (long long)i + (long long)pitch * j : 0;
+    if constexpr (INT) {
+      inside[u] = k < WZ * WR;
+      // the tail of the last pass loads the window's first point again
+      const uint32_t boff = inside[u] ? 8u * (uint32_t)(i + pitch * j) : 8u * (uint32_t)(i0 - HALO + pitch * (j0 - HALO));
+      const Rsrc rq = rsrc(a.qin);
 #pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      const double x = a.qin[gi + f * fs];
-      win[u][f] = inside[u] ? x : 0.0;
-    }
-    if constexpr (LAD) {
-      wlad[u][0] = a.lad[gi];
-      wlad[u][1] = a.lad[gi + 2 * fs];
+      for (int f = 0; f < 5; ++f) win[u][f] = bld(rq, boff, f * fsb);
+      if constexpr (LAD) {
+        const Rsrc rl = rsrc(a.lad);
+        wlad[u][0] = bld(rl, boff, 0);
+        wlad[u][1] = bld(rl, boff, 2 * fsb);
+      }
+    } else {
+      inside[u] = k < WZ * WR && i >= 0 && i < nz && j >= 0 && j < nr;
+      // points outside the grid load point 0 (a valid address) and are zeroed after the load,
+      // so every load issues without a branch
+      const long long gi = inside[u] ? (long long)i + (long long)pitch * j : 0;
+#pragma unroll
+      for (int f = 0; f < 5; ++f) {
+        const double x = a.qin[gi + f * fs];
+        win[u][f] = inside[u] ? x : 0.0;
+      }
+      if constexpr (LAD) {
+        wlad[u][0] = a.lad[gi];
+        wlad[u][1] = a.lad[gi + 2 * fs];
+      }
     }
   }
 #pragma unroll
@@ -143,9 +186,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     if (k >= WZ * WR) break;
     const int wz = k % WZ, wr = k / WZ;
     double p = 0.0, mu = 0.0, kap = 0.0, mphys = 0.0;
-    if (inside[u]) {
+    if (INT || inside[u]) {
       p = win[u][0] * win[u][4];
-      mu = pow(win[u][4], 0.666);
+      mu = INT ? pow0666(win[u][4]) : pow(win[u][4], 0.666);
       mphys = mu;
       kap = mu;
       if constexpr (LAD) {
@@ -170,8 +213,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     const int tz = k % TZ, wr = k / TZ;
     const int i = i0 + tz, j = j0 - HALO + wr;
     double dvr = 0.0, dvz = 0.0;
-    if (i >= oi_lo && i < oi_hi && j >= 0 && j < nr) {
-      const uint32_t zr = INT ? ROW_INTERIOR : (a.meta[(long long)i + (long long)pitch * j] & 15u);
+    if (INT) {
+      if (i < oi_hi) {
+        dvr = d1i(&sF[1][wr][tz + HALO], 1) * mcol;
+        dvz = d1i(&sF[3][wr][tz + HALO], 1) * mcol;
+      }
+    } else if (i >= oi_lo && i < oi_hi && j >= 0 && j < nr) {
+      const uint32_t zr = a.meta[(long long)i + (long long)pitch * j] & 15u;
       dvr = d1_sum(zr, &sF[1][wr][tz + HALO], 1) * mcol;
       dvz = d1_sum(zr, &sF[3][wr][tz + HALO], 1) * mcol;
     }
@@ -219,6 +267,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     if (i < oi_lo || i >= oi_hi || j < oj_lo || j >= oj_hi) continue;
     const int wz = tz + HALO, wr = ty + HALO;
     const long long gi = (long long)i + (long long)pitch * j;
+    const uint32_t boff = 8u * (uint32_t)(i + pitch * j);
     const uint32_t meta = INT ? ROW_INTERIOR | (ROW_INTERIOR << 4) : a.meta[gi];
     const uint32_t zr = INT ? ROW_INTERIOR : meta & 15u, rr = INT ? ROW_INTERIOR : (meta >> 4) & 15u;
     const double mz = mcol, mz2 = mcol2;  // the column's metrics, loaded with the window
@@ -231,12 +280,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     if (STAGE >= 2) {
       const double* __restrict__ src0 = STAGE == 4 ? a.acc_a : a.qn;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) pin0[k] = src0[gi + k * fs];
+      for (int k = 0; k < 5; ++k) pin0[k] = INT ? bld(rsrc(src0), boff, k * fsb) : src0[gi + k * fs];
     }
     if (STAGE >= 3) {
       const double* __restrict__ src1 = a.acc_s;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) pin1[k] = src1[gi + k * fs];
+      for (int k = 0; k < 5; ++k) pin1[k] = INT ? bld(rsrc(src1), boff, k * fsb) : src1[gi + k * fs];
     }
     // the target where a boundary patch or a sponge needs it
     const uint32_t sid = INT ? 0u : (meta >> 8) & 255u;
@@ -245,7 +294,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     double qt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (sid != 0 || top || inflow || (FORCE && (a.forcing & AJET_FORCING_FEEDBACK))) {
 #pragma unroll
-      for (int f = 0; f < 5; ++f) qt[f] = a.tgt[gi + f * fs];
+      for (int f = 0; f < 5; ++f) qt[f] = INT ? bld(rsrc(a.tgt), boff, f * fsb) : a.tgt[gi + f * fs];
     }
 
     double q[5];
@@ -253,69 +302,118 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     for (int f = 0; f < 5; ++f) q[f] = sF[f][wr][wz];
 
     PointDerivs d;
-    d.drhodz = d1_sum(zr, &sF[0][wr][wz], 1) * mz;
-    d.dVrdz = sD[0][wr][tz];
-    d.dVtdz = d1_sum(zr, &sF[2][wr][wz], 1) * mz;
-    d.dVzdz = sD[1][wr][tz];
-    d.dTdz = d1_sum(zr, &sF[4][wr][wz], 1) * mz;
-    d.dpdz = d1_sum(zr, &sF[5][wr][wz], 1) * mz;
-    AJET_FENCE();
-    d.drhodr = d1_sum(rr, &sF[0][wr][wz], WZ) * mr;
-    d.dVrdr = d1_sum(rr, &sF[1][wr][wz], WZ) * mr;
-    d.dVtdr = d1_sum(rr, &sF[2][wr][wz], WZ) * mr;
-    d.dVzdr = d1_sum(rr, &sF[3][wr][wz], WZ) * mr;
-    d.dTdr = d1_sum(rr, &sF[4][wr][wz], WZ) * mr;
-    d.dpdr = d1_sum(rr, &sF[5][wr][wz], WZ) * mr;
-    AJET_FENCE();
-
     double rhs[5];
-    if (!INT && j == 0) {
-      // the axis row is skipped by the interior loops (ModRHS.f90:59-67): rhs = -0
-#pragma unroll
-      for (int k = 0; k < 5; ++k) rhs[k] = -1.0 * 0.0;
-    } else {
-      inviscid(q, r, d, a.pc.gam, rhs);
-    }
-    AJET_FENCE();
     double mu = sF[6][wr][wz];
-    if (VISC) {
-      d.dmudz = d1_sum(zr, &sF[6][wr][wz], 1) * mz;
-      d.dmudr = d1_sum(rr, &sF[6][wr][wz], WZ) * mr;
-      double kappa = mu;
-      if constexpr (LAD) {
-        kappa = sF[FK][wr][wz];
-        d.dkappadz = d1_sum(zr, &sF[FK][wr][wz], 1) * mz;
-        d.dkappadr = d1_sum(rr, &sF[FK][wr][wz], WZ) * mr;
-      } else {
-        d.dkappadz = d.dmudz;
-        d.dkappadr = d.dmudr;
-      }
-      const double z2 = mz * mz, r2 = mr * mr;
-      AJET_FENCE();
-      d.d2Vrdz2 = d2_sum(zr, &sF[1][wr][wz], 1) * z2 + d.dVrdz * mz2;
-      d.d2Vtdz2 = d2_sum(zr, &sF[2][wr][wz], 1) * z2 + d.dVtdz * mz2;
-      d.d2Vzdz2 = d2_sum(zr, &sF[3][wr][wz], 1) * z2 + d.dVzdz * mz2;
-      d.d2Tdz2 = d2_sum(zr, &sF[4][wr][wz], 1) * z2 + d.dTdz * mz2;
-      AJET_FENCE();
-      d.d2Vrdr2 = d2_sum(rr, &sF[1][wr][wz], WZ) * r2 + d.dVrdr * mr2;
-      d.d2Vtdr2 = d2_sum(rr, &sF[2][wr][wz], WZ) * r2 + d.dVtdr * mr2;
-      d.d2Vzdr2 = d2_sum(rr, &sF[3][wr][wz], WZ) * r2 + d.dVzdr * mr2;
-      d.d2Tdr2 = d2_sum(rr, &sF[4][wr][wz], WZ) * r2 + d.dTdr * mr2;
-      AJET_FENCE();
-      d.d2Vrdzdr = d1_sum(rr, &sD[0][wr][tz], TZ) * mr;
-      d.d2Vzdzdr = d1_sum(rr, &sD[1][wr][tz], TZ) * mr;
-      AJET_FENCE();
-      if (INT || j != 0) {
+    if constexpr (INT) {
+      const double ir = a.rinv[j];
+      const double irho = rcp(q[0]);
+      d.drhodz = d1i(&sF[0][wr][wz], 1) * mz;
+      d.dVrdz = sD[0][wr][tz];
+      d.dVtdz = d1i(&sF[2][wr][wz], 1) * mz;
+      d.dVzdz = sD[1][wr][tz];
+      d.dTdz = d1i(&sF[4][wr][wz], 1) * mz;
+      d.dpdz = d1i(&sF[5][wr][wz], 1) * mz;
+      d.drhodr = d1i(&sF[0][wr][wz], WZ) * mr;
+      d.dVrdr = d1i(&sF[1][wr][wz], WZ) * mr;
+      d.dVtdr = d1i(&sF[2][wr][wz], WZ) * mr;
+      d.dVzdr = d1i(&sF[3][wr][wz], WZ) * mr;
+      d.dTdr = d1i(&sF[4][wr][wz], WZ) * mr;
+      d.dpdr = d1i(&sF[5][wr][wz], WZ) * mr;
+      inviscid_fast(q, ir, irho, d, a.pc, rhs);
+      if (VISC) {
+        d.dmudz = d1i(&sF[6][wr][wz], 1) * mz;
+        d.dmudr = d1i(&sF[6][wr][wz], WZ) * mr;
+        double kappa = mu;
+        if constexpr (LAD) {
+          kappa = sF[FK][wr][wz];
+          d.dkappadz = d1i(&sF[FK][wr][wz], 1) * mz;
+          d.dkappadr = d1i(&sF[FK][wr][wz], WZ) * mr;
+        } else {
+          d.dkappadz = d.dmudz;
+          d.dkappadr = d.dmudr;
+        }
+        const double z2 = mz * mz, r2 = mr * mr;
+        d.d2Vrdz2 = fma(d2i(&sF[1][wr][wz], 1), z2, d.dVrdz * mz2);
+        d.d2Vtdz2 = fma(d2i(&sF[2][wr][wz], 1), z2, d.dVtdz * mz2);
+        d.d2Vzdz2 = fma(d2i(&sF[3][wr][wz], 1), z2, d.dVzdz * mz2);
+        d.d2Tdz2 = fma(d2i(&sF[4][wr][wz], 1), z2, d.dTdz * mz2);
+        d.d2Vrdr2 = fma(d2i(&sF[1][wr][wz], WZ), r2, d.dVrdr * mr2);
+        d.d2Vtdr2 = fma(d2i(&sF[2][wr][wz], WZ), r2, d.dVtdr * mr2);
+        d.d2Vzdr2 = fma(d2i(&sF[3][wr][wz], WZ), r2, d.dVzdr * mr2);
+        d.d2Tdr2 = fma(d2i(&sF[4][wr][wz], WZ), r2, d.dTdr * mr2);
+        d.d2Vrdzdr = d1i(&sD[0][wr][tz], TZ) * mr;
+        d.d2Vzdzdr = d1i(&sD[1][wr][tz], TZ) * mr;
         double lambda = 0.0;
         if (SC) {
-          // beta = (3/5) T**0.666 (+ beta*), lambda = beta - (2/3) mu (ModAssign.f90:21-23,
-          // ModShockCapturing.f90:249-250); without the artificial terms mu is T**0.666 itself
           const double mu_phys = LAD ? sF[FM][wr][wz] : mu;
-          double beta = (3.0 / 5.0) * mu_phys;
-          if (LAD) beta = beta + a.lad[gi + fs];
-          lambda = beta - (2.0 / 3.0) * mu;
+          double beta = 0.6 * mu_phys;
+          if (LAD) beta = beta + bld(rsrc(a.lad), boff, fsb);
+          lambda = fma(-2.0 / 3.0, mu, beta);
         }
-        viscous(q, r, mu, kappa, lambda, SC, d, a.pc, rhs);
+        viscous_fast(q, ir, irho, mu, kappa, lambda, SC, d, a.pc, rhs);
+      }
+    } else {
+      d.drhodz = d1_sum(zr, &sF[0][wr][wz], 1) * mz;
+      d.dVrdz = sD[0][wr][tz];
+      d.dVtdz = d1_sum(zr, &sF[2][wr][wz], 1) * mz;
+      d.dVzdz = sD[1][wr][tz];
+      d.dTdz = d1_sum(zr, &sF[4][wr][wz], 1) * mz;
+      d.dpdz = d1_sum(zr, &sF[5][wr][wz], 1) * mz;
+      AJET_FENCE();
+      d.drhodr = d1_sum(rr, &sF[0][wr][wz], WZ) * mr;
+      d.dVrdr = d1_sum(rr, &sF[1][wr][wz], WZ) * mr;
+      d.dVtdr = d1_sum(rr, &sF[2][wr][wz], WZ) * mr;
+      d.dVzdr = d1_sum(rr, &sF[3][wr][wz], WZ) * mr;
+      d.dTdr = d1_sum(rr, &sF[4][wr][wz], WZ) * mr;
+      d.dpdr = d1_sum(rr, &sF[5][wr][wz], WZ) * mr;
+      AJET_FENCE();
+      if (j == 0) {
+        // the axis row is skipped by the interior loops (ModRHS.f90:59-67): rhs = -0
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rhs[k] = -1.0 * 0.0;
+      } else {
+        inviscid(q, r, d, a.pc.gam, rhs);
+      }
+      AJET_FENCE();
+      if (VISC) {
+        d.dmudz = d1_sum(zr, &sF[6][wr][wz], 1) * mz;
+        d.dmudr = d1_sum(rr, &sF[6][wr][wz], WZ) * mr;
+        double kappa = mu;
+        if constexpr (LAD) {
+          kappa = sF[FK][wr][wz];
+          d.dkappadz = d1_sum(zr, &sF[FK][wr][wz], 1) * mz;
+          d.dkappadr = d1_sum(rr, &sF[FK][wr][wz], WZ) * mr;
+        } else {
+          d.dkappadz = d.dmudz;
+          d.dkappadr = d.dmudr;
+        }
+        const double z2 = mz * mz, r2 = mr * mr;
+        AJET_FENCE();
+        d.d2Vrdz2 = d2_sum(zr, &sF[1][wr][wz], 1) * z2 + d.dVrdz * mz2;
+        d.d2Vtdz2 = d2_sum(zr, &sF[2][wr][wz], 1) * z2 + d.dVtdz * mz2;
+        d.d2Vzdz2 = d2_sum(zr, &sF[3][wr][wz], 1) * z2 + d.dVzdz * mz2;
+        d.d2Tdz2 = d2_sum(zr, &sF[4][wr][wz], 1) * z2 + d.dTdz * mz2;
+        AJET_FENCE();
+        d.d2Vrdr2 = d2_sum(rr, &sF[1][wr][wz], WZ) * r2 + d.dVrdr * mr2;
+        d.d2Vtdr2 = d2_sum(rr, &sF[2][wr][wz], WZ) * r2 + d.dVtdr * mr2;
+        d.d2Vzdr2 = d2_sum(rr, &sF[3][wr][wz], WZ) * r2 + d.dVzdr * mr2;
+        d.d2Tdr2 = d2_sum(rr, &sF[4][wr][wz], WZ) * r2 + d.dTdr * mr2;
+        AJET_FENCE();
+        d.d2Vrdzdr = d1_sum(rr, &sD[0][wr][tz], TZ) * mr;
+        d.d2Vzdzdr = d1_sum(rr, &sD[1][wr][tz], TZ) * mr;
+        AJET_FENCE();
+        if (j != 0) {
+          double lambda = 0.0;
+          if (SC) {
+            // beta = (3/5) T**0.666 (+ beta*), lambda = beta - (2/3) mu (ModAssign.f90:21-23,
+            // ModShockCapturing.f90:249-250); without the artificial terms mu is T**0.666 itself
+            const double mu_phys = LAD ? sF[FM][wr][wz] : mu;
+            double beta = (3.0 / 5.0) * mu_phys;
+            if (LAD) beta = beta + a.lad[gi + fs];
+            lambda = beta - (2.0 / 3.0) * mu;
+          }
+          viscous(q, r, mu, kappa, lambda, SC, d, a.pc, rhs);
+        }
       }
     }
 
@@ -364,6 +462,45 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     // ---- stage update (ModRungeKutta.f90:32-146), from the inputs loaded at the top
     double out[5], acc[5];
     const double c16 = 1.0 / 6.0, c13 = 1.0 / 3.0;
+    if constexpr (INT) {
+      // k = dt rhs folded into one fused multiply-add per value
+      const double hdt = 0.5 * dt, dt6 = c16 * dt;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        if (STAGE == 1) {
+          out[k] = fma(hdt, rhs[k], q[k]);
+          acc[k] = fma(dt6, rhs[k], q[k]);
+        } else if (STAGE == 2) {
+          out[k] = fma(hdt, rhs[k], pin0[k]);
+          acc[k] = dt * rhs[k];
+        } else if (STAGE == 3) {
+          out[k] = fma(dt, rhs[k], pin0[k]);
+          acc[k] = fma(dt, rhs[k], pin1[k]);
+        } else {
+          out[k] = fma(dt6, rhs[k], fma(c13, pin1[k], pin0[k]));
+        }
+      }
+      if (STAGE == 1) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) bst(rsrc(a.acc_a), boff, k * fsb, acc[k]);
+      } else if (STAGE == 2 || STAGE == 3) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) bst(rsrc(a.acc_s), boff, k * fsb, acc[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < 5; ++k) bst(rsrc(a.qout), boff, k * fsb, out[k]);
+      if (STAGE == 4) {
+        // ComputeTimeStep's denominator of the new state (lambda / Re never wins: lambda < 0)
+        const double T = out[4];
+        const double mun = pow0666(T);
+        const double inv = fabs(fma(out[3], mz, out[1] * mr)) + sTst * sqrt(fma(mz, mz, mr * mr));
+        const double sm = fabs(mz) + fabs(mr);
+        const double den = fma(2.0 * (mun * a.pc.cfl_visc), sm * sm, inv);
+        if (!(den <= 1.0e300)) a.sc->nonfinite = 1;
+        dmax = fmax(dmax, den);
+      }
+      continue;
+    }
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       const double kk = dt * rhs[k];
@@ -389,12 +526,12 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
 #pragma unroll
       for (int k = 0; k < 5; ++k) accS[gi + k * fs] = acc[k];
     }
-    if (STAGE == 4 && !INT && zr == ROW_BLANK) {
+    if (STAGE == 4 && zr == ROW_BLANK) {
       // blanked points skip the final update and keep the last stage value (ModRungeKutta.f90:139)
 #pragma unroll
       for (int k = 0; k < 5; ++k) out[k] = q[k];
     }
-    if (!INT && j == 0) {
+    if (j == 0) {
       // axis_conditions (ModAxis.f90:14-16) from the stage input rows 2..6
       const double a00 = c_sbp1[0][0];
       const int fl[3] = {0, 3, 4};
@@ -409,7 +546,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
 #pragma unroll
     for (int k = 0; k < 5; ++k) a.qout[gi + k * fs] = out[k];
 
-    if (STAGE == 4 && (INT || zr != ROW_BLANK)) {
+    if (STAGE == 4 && zr != ROW_BLANK) {
       const double den = cfl_denominator(out[1], out[3], out[4], sTst, mz, mr, a.pc.Re, a.pc.Pr);
       if (!(den <= 1.0e300)) a.sc->nonfinite = 1;
       dmax = fmax(dmax, den);

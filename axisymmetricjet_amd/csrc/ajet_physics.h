@@ -1,12 +1,14 @@
 // Pointwise physics of ComputeRHS (ModRHS.f90:5-258) and the boundary conditions
-// (ModBC.f90:16-617) as device functions.  Every expression keeps the reference's
-// left-to-right association; the library is built with -ffp-contract=off, so each product
-// and sum is rounded on its own exactly as in the reference's unoptimised build.
+// (ModBC.f90:16-617) as device functions.  inviscid, viscous and the boundary conditions keep
+// the reference's left-to-right association; the library is built with -ffp-contract=off, so
+// each product and sum is rounded on its own exactly as in the reference's unoptimised build.
+// inviscid_fast and viscous_fast are the interior tiles' fused forms.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include "ajet_internal.h"
+#include "ajet_fastmath.h"
 
 namespace ajet {
 
@@ -23,6 +25,9 @@ struct PhysConst {
   double gam, Re, Pr;
   double sponge_start_top, sponge_A_top, sponge_end_left, sponge_A_left;
   double r_top;  // rvec(i, Nr) of the tensor-product grid
+  // products of the constants, for the interior-tile forms (viscous_fast and the CFL rule)
+  double inv_gam, gm1, inv_Re, g_RePr, g_gm1_Re;  // 1/gam, gam - 1, 1/Re, gam/(Re Pr), gam (gam - 1)/Re
+  double cfl_visc;                                 // max(1/Re, 1/(Re Pr))
 };
 
 // Inviscid terms, ModRHS.f90:92-96, already negated (ModRHS.f90:103-113).
@@ -73,6 +78,52 @@ __device__ __forceinline__ void viscous(const double q[5], double r, double mu, 
                 Vr * onebyr * (fourby3 * Vr * onebyr - twoby3 * d.dVrdr - twoby3 * d.dVzdz) + d.dVtdz * d.dVtdz +
                 d.dVzdz * (fourby3 * d.dVzdz - twoby3 * d.dVrdr - twoby3 * Vr * onebyr) +
                 (d.dVrdz + d.dVzdr) * (d.dVzdr + d.dVrdz));
+}
+
+// The same inviscid and viscous terms for interior tiles, factored around fused multiply-adds
+// with one reciprocal of rho and the row's 1/r (ir) from a table; the association differs from
+// the reference's, so the results differ by a few ulp of the terms.
+__device__ __forceinline__ void inviscid_fast(const double q[5], double ir, double irho, const PointDerivs& d,
+                                              const PhysConst& c, double rhs[5]) {
+  const double rho = q[0], Vr = q[1], Vt = q[2], Vz = q[3], T = q[4];
+  const double div = d.dVrdr + fma(Vr, ir, d.dVzdz);  // dVr/dr + Vr/r + dVz/dz
+  const double igr = irho * c.inv_gam;                 // 1/(gam rho)
+  rhs[0] = -fma(Vr, d.drhodr, fma(Vz, d.drhodz, rho * div));
+  rhs[1] = -fma(Vr, d.dVrdr, fma(Vz, d.dVrdz, igr * d.dpdr));
+  rhs[2] = -fma(Vr, fma(Vt, ir, d.dVtdr), Vz * d.dVtdz);
+  rhs[3] = -fma(Vr, d.dVzdr, fma(Vz, d.dVzdz, igr * d.dpdz));
+  rhs[4] = -fma(Vr, d.dTdr, fma(Vz, d.dTdz, (c.gm1 * T) * div));
+}
+
+// ModRHS.f90:180-205 with vri = Vr/r and the three strain combinations shared between the
+// momentum and the dissipation terms.
+__device__ __forceinline__ void viscous_fast(const double q[5], double ir, double irho, double mu, double kappa,
+                                             double lambda, bool sc, const PointDerivs& d, const PhysConst& c,
+                                             double rhs[5]) {
+  const double Vr = q[1], Vt = q[2];
+  double f43 = 29.0 / 15.0, f13 = 14.0 / 15.0, f23 = 1.0 / 15.0;
+  if (sc) {
+    const double tmp = lambda * rcp(mu);
+    f43 = 2.0 + tmp;
+    f13 = 1.0 + tmp;
+    f23 = -tmp;
+  }
+  const double vri = Vr * ir;
+  const double sA = fma(f43, d.dVrdr, -f23 * (vri + d.dVzdz));   // f43 Vr_r - f23 Vr/r - f23 Vz_z
+  const double sB = fma(f43, vri, -f23 * (d.dVrdr + d.dVzdz));   // f43 Vr/r - f23 Vr_r - f23 Vz_z
+  const double sC = fma(f43, d.dVzdz, -f23 * (d.dVrdr + vri));   // f43 Vz_z - f23 Vr_r - f23 Vr/r
+  const double shear = d.dVrdz + d.dVzdr;
+  const double w = fma(-Vt, ir, d.dVtdr);                         // Vt_r - Vt/r
+  const double a = irho * c.inv_Re;                               // 1/(Re rho)
+  rhs[1] = fma(a, fma(mu, fma(f43, fma(ir, d.dVrdr - vri, d.d2Vrdr2), fma(f13, d.d2Vzdzdr, d.d2Vrdz2)),
+                      fma(d.dmudr, sA, d.dmudz * shear)), rhs[1]);
+  rhs[2] = fma(a, fma(mu, fma(ir, w, d.d2Vtdr2 + d.d2Vtdz2), fma(d.dmudr, w, d.dmudz * d.dVtdz)), rhs[2]);
+  rhs[3] = fma(a, fma(mu, d.d2Vzdr2 + fma(f43, d.d2Vzdz2, fma(f13, fma(ir, d.dVrdz, d.d2Vrdzdr), ir * d.dVzdr)),
+                      fma(d.dmudr, shear, d.dmudz * sC)), rhs[3]);
+  const double heat = fma(kappa, d.d2Tdr2 + fma(ir, d.dTdr, d.d2Tdz2), fma(d.dkappadr, d.dTdr, d.dkappadz * d.dTdz));
+  const double phi = fma(d.dVrdr, sA, fma(w, d.dVtdr - vri, fma(vri, sB, fma(d.dVtdz, d.dVtdz,
+                     fma(d.dVzdz, sC, shear * shear)))));
+  rhs[4] = fma(c.g_RePr * irho, heat, fma(c.g_gm1_Re * mu * irho, phi, rhs[4]));
 }
 
 // Characteristic matrices of eigen_matrices_in_z_direction / _in_r_direction

@@ -273,7 +273,8 @@ void classify_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, 
   for (int tr = 0; tr < ntr; ++tr)
     for (int tz = 0; tz < ntz; ++tz) {
       const int i0 = s.zcuts[tz], i1 = s.zcuts[tz + 1], j0 = s.rcuts[tr], j1 = s.rcuts[tr + 1];
-      bool ok = j0 > 0;
+      // the window (HALO points on every side) lies inside the grid
+      bool ok = j0 - HALO >= 0 && j1 + HALO <= nr && i0 - HALO >= 0 && i1 + HALO <= nz;
       for (int j = std::max(0, j0 - HALO); ok && j < std::min(nr, j1 + HALO); ++j)
         for (int i = i0; ok && i < i1; ++i) {
           const uint32_t m = s.meta[(size_t)i + (size_t)nz * j];

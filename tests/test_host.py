@@ -110,3 +110,17 @@ def test_mixed_tile_list(tmp_path):
                            os.path.join(ROOT, "axisymmetricjet_amd", "csrc", "ajet_setup.cpp"), "-o", exe])
     out = subprocess.run([exe], capture_output=True, text=True)
     assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
+
+
+def test_interior_fast_math(tmp_path):
+    """The interior tiles' FP64 helpers (csrc/ajet_fastmath.h), built for the host with hipcc:
+    T**0.666 within 1e-15 of the C library's pow over its whole fast range (2^-8, 2^8) and the
+    library's pow itself outside it; the reciprocal within an ulp; the factored D1 and D2
+    interior stencils against the reference's unfused sums and exact on polynomials."""
+    exe = str(tmp_path / "fastmath_check")
+    subprocess.check_call(["/opt/rocm/bin/hipcc", "-x", "hip", "--offload-arch=gfx950", "-O2", "-std=c++17",
+                           "-ffp-contract=off", "-I", os.path.join(ROOT, "axisymmetricjet_amd", "csrc"),
+                           "-I", os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "fastmath_check.cpp"),
+                           "-o", exe], stderr=subprocess.DEVNULL)
+    out = subprocess.run([exe], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
