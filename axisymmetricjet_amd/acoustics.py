@@ -26,20 +26,35 @@ def arc_points(zc, rc, centre=15.0, radius=6.5, angles=ARC_ANGLES):
 
 def record(ctx, nsteps, every=2):
     """Advance ctx by nsteps (a multiple of `every`), sampling the probes' pressure after each
-    `every` steps: returns times (m,) and pressures (m, nprobes)."""
+    `every` steps: returns times (m,) and pressures (m, nprobes) since the first call's start.
+    Stops at the first sample that is not finite (or a step whose time-step reduction meets a
+    non-finite value), returning the finite samples before it."""
     times, ps = [], []
     for _ in range(nsteps // every):
-        ctx.run(every)
-        v = ctx.probe()
-        ps.append(v[:, 0] * v[:, 4])
-        times.append(ctx.dt_log(0)[1])
-    return np.array(times), np.array(ps)
+        try:
+            ctx.run(every)
+            v = ctx.probe()
+            t = ctx.dt_log(0)[1]
+        except RuntimeError:
+            break
+        p = v[:, 0] * v[:, 4]
+        if not (np.all(np.isfinite(p)) and np.isfinite(t)):
+            break
+        ps.append(p)
+        times.append(t)
+    return np.array(times), np.array(ps).reshape(len(times), -1)
 
 
 def oaspl(p):
     """OASPL per probe of pressure histories p (m, nprobes), in dB re the ambient pressure."""
     pf = p - p.mean(axis=0, keepdims=True)
     return 10.0 * np.log10(np.mean(pf * pf, axis=0) + 1e-300)
+
+
+def lowest_strouhal(t, uj, nseg=4, d=1.0):
+    """The lowest Strouhal number f D / Uj that spectrum() resolves over a record of times t:
+    its segments span a quarter (1/nseg) of the record, so it is nseg D / (Uj T)."""
+    return nseg * d / (uj * (t[-1] - t[0]))
 
 
 def spectrum(t, p, nseg=4):

@@ -25,7 +25,9 @@ CONTAINS
     CHARACTER(LEN=*), INTENT(IN) :: fname
     INTEGER(INT64), INTENT(INOUT) :: pos
     INTEGER(INT32) :: m
-    READ (u, POS=pos) m
+    INTEGER :: ios
+    READ (u, POS=pos, IOSTAT=ios) m
+    IF (ios /= 0) CALL fail('file ends inside a record:', fname)
     IF (INT(m, INT64) /= nbytes) CALL fail('unexpected record length in', fname)
     pos = pos + 4
   END SUBROUTINE open_record
@@ -36,7 +38,9 @@ CONTAINS
     CHARACTER(LEN=*), INTENT(IN) :: fname
     INTEGER(INT64), INTENT(INOUT) :: pos
     INTEGER(INT32) :: m
-    READ (u, POS=pos + nbytes) m
+    INTEGER :: ios
+    READ (u, POS=pos + nbytes, IOSTAT=ios) m
+    IF (ios /= 0) CALL fail('file ends inside a record:', fname)
     IF (INT(m, INT64) /= nbytes) CALL fail('record marker mismatch in', fname)
     pos = pos + nbytes + 4
   END SUBROUTINE close_record
@@ -47,12 +51,15 @@ CONTAINS
     INTEGER(INT64), INTENT(INOUT) :: pos
     INTEGER, INTENT(OUT) :: nz, nr
     INTEGER(INT32) :: ngrid, nd(3)
+    INTEGER :: ios
     CALL open_record(u, 4_INT64, fname, pos)
-    READ (u, POS=pos) ngrid
+    READ (u, POS=pos, IOSTAT=ios) ngrid
+    IF (ios /= 0) CALL fail('file ends inside a record:', fname)
     CALL close_record(u, 4_INT64, fname, pos)
     IF (ngrid /= 1) CALL fail('only single-block files are supported:', fname)
     CALL open_record(u, 12_INT64, fname, pos)
-    READ (u, POS=pos) nd
+    READ (u, POS=pos, IOSTAT=ios) nd
+    IF (ios /= 0) CALL fail('file ends inside a record:', fname)
     CALL close_record(u, 12_INT64, fname, pos)
     IF (nd(3) /= 1) CALL fail('expected ND(1,3) = 1 in', fname)
     nz = nd(1)
@@ -77,16 +84,19 @@ CONTAINS
     pos = 1
     CALL read_header(u, fname, pos, nz, nr)
     npts = INT(nz, INT64) * nr
-    READ (u, POS=pos) m
+    READ (u, POS=pos, IOSTAT=ios) m
+    IF (ios /= 0) CALL fail('file ends inside a record:', fname)
     nb = INT(m, INT64)
     ALLOCATE (z(nz, nr), r(nz, nr), w(nz, nr), iblank(nz, nr))
     IF (nb == 3 * npts * 8 + npts * 4) THEN
       pos = pos + 4
-      READ (u, POS=pos) z, r, w, iblank
+      READ (u, POS=pos, IOSTAT=ios) z, r, w, iblank
+      IF (ios /= 0) CALL fail('file ends inside a record:', fname)
       CALL close_record(u, nb, fname, pos)
     ELSE IF (nb == 3 * npts * 8) THEN
       pos = pos + 4
-      READ (u, POS=pos) z, r, w
+      READ (u, POS=pos, IOSTAT=ios) z, r, w
+      IF (ios /= 0) CALL fail('file ends inside a record:', fname)
       CALL close_record(u, nb, fname, pos)
       iblank = 1
     ELSE
@@ -111,12 +121,14 @@ CONTAINS
     pos = 1
     CALL read_header(u, fname, pos, nz, nr)
     CALL open_record(u, 32_INT64, fname, pos)
-    READ (u, POS=pos) tau
+    READ (u, POS=pos, IOSTAT=ios) tau
+    IF (ios /= 0) CALL fail('file ends inside a record:', fname)
     CALL close_record(u, 32_INT64, fname, pos)
     ALLOCATE (q(nz, nr, 5))
     nb = INT(nz, INT64) * nr * 5 * 8
     CALL open_record(u, nb, fname, pos)
-    READ (u, POS=pos) q
+    READ (u, POS=pos, IOSTAT=ios) q
+    IF (ios /= 0) CALL fail('file ends inside a record:', fname)
     CALL close_record(u, nb, fname, pos)
     CLOSE (u)
   END SUBROUTINE p3d_read_soln

@@ -208,7 +208,10 @@ def write_soln(path: str, q: np.ndarray, tau=(0.0, 0.0, 0.0, 0.0)) -> None:
 
 
 def _records(path: str):
-    data = open(path, "rb").read()
+    return _records_of(open(path, "rb").read(), path)
+
+
+def _records_of(data: bytes, path: str = "data"):
     pos, out = 0, []
     while pos < len(data):
         (n,) = struct.unpack_from("<i", data, pos)
@@ -238,7 +241,15 @@ def read_grid(path: str):
 
 
 def read_soln(path: str):
-    recs = _records(path)
+    return _soln_of(_records(path))
+
+
+def read_soln_bytes(data: bytes):
+    """read_soln of a file's contents."""
+    return _soln_of(_records_of(data))
+
+
+def _soln_of(recs):
     nz, nr, nk = struct.unpack("<3i", recs[1])
     tau = struct.unpack("<4d", recs[2])
     q = np.frombuffer(recs[3], "<f8").reshape((nz, nr, 5), order="F")

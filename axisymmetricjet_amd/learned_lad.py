@@ -160,56 +160,81 @@ def zero_model():
 
 
 def score(case, model, sensor_threshold, steps=2000, every=4, ramp_min=1e-8, refresh=1,
-          variants=("learned", "zero")):
+          variants=("learned", "zero"), warm=0, uj=None):
     """Runs of the case under the reference's LAD and under the learned mode (and the
-    zero-coefficient mode), with pressure histories on the arc of probes: OASPL and spectrum
-    differences from the reference run per probe, the compression peaks along the axis and the
-    lip line, the fallback fraction and whether each run stayed finite."""
+    zero-coefficient mode), all from the same state -- the case's initial condition advanced
+    `warm` steps under the reference's LAD, so that the record starts on a developed jet -- with
+    pressure histories on the arc of probes over `steps` steps: OASPL and spectrum differences
+    from the reference run per probe over the record the runs share (a run stops at its first
+    non-finite sample), the compression peaks along the axis and the lip line, the fallback
+    fraction, the record length in acoustic time units (D / c_inf, c_inf = 1) and the lowest
+    Strouhal number the spectra resolve (uj: the jet velocity)."""
     from . import acoustics as ac
 
     zc, rc = case.z[:, 0], case.r[0, :]
     pts = ac.arc_points(zc, rc)
     j_lip = int(np.argmin(np.abs(rc - 0.5)))
+    start = case.q0
+    if warm > 0:
+        w = capi.Context(case, **LAD)
+        w.run(warm)
+        start = w.state()
+        w.close()
     runs = {}
     for name in ("lad",) + tuple(variants):
         ctx = capi.Context(case, **LAD)
+        ctx.set_state(start, case.target)
         if name != "lad":
             ctx.set_learned_lad(model if name == "learned" else zero_model(), sensor_threshold, ramp_min, refresh)
         ctx.set_probes(pts)
-        rec = {"finite": True}
+        t, p = ac.record(ctx, steps, every)
+        rec = {"samples": len(t), "t": t, "p": p}
         try:
-            t, p = ac.record(ctx, steps, every)
             q = ctx.state()
-            rec["finite"] = bool(np.isfinite(q).all() and np.isfinite(p).all())
-        except capi.AjetError as e:
-            rec["finite"], rec["error"] = False, str(e)
-        if rec["finite"]:
-            rec.update(t=t, p=p, oaspl=ac.oaspl(p), spec=ac.spectrum(t, p),
-                       shocks={"axis": ac.shock_positions(q, zc, 0), "lip": ac.shock_positions(q, zc, j_lip)})
-            if name != "lad":
+            rec["q"] = q if np.isfinite(q).all() else None
+        except capi.AjetError:
+            rec["q"] = None
+        if name != "lad":
+            try:
                 rec["fallback"] = ctx.fallback_fraction()
+            except capi.AjetError:
+                rec["fallback"] = None
         runs[name] = rec
         ctx.close()
-    ref = runs["lad"]
-    out = {"steps": steps, "probe_every": every, "probes": len(pts), "sensor_threshold": sensor_threshold,
-           "ramp_min": ramp_min, "refresh": refresh, "reference_finite": ref["finite"]}
-    if not ref["finite"]:
+    m = min(r["samples"] for r in runs.values())
+    out = {"steps": steps, "warm_steps": warm, "probe_every": every, "probes": len(pts),
+           "sensor_threshold": sensor_threshold, "ramp_min": ramp_min, "refresh": refresh,
+           "finite_steps": {k: r["samples"] * every for k, r in runs.items()},
+           "all_finite": all(r["samples"] * every == steps for r in runs.values())}
+    if m < 16:
         return out
-    out["oaspl_lad_db"] = [float(x) for x in ref["oaspl"]]
-    out["shocks_lad"] = {"axis": len(ref["shocks"]["axis"]), "lip": len(ref["shocks"]["lip"])}
+    ref = runs["lad"]
+    t = ref["t"][:m]
+    out["record_time_D_per_c"] = float(t[-1] - t[0])
+    if uj:
+        out["lowest_strouhal"] = float(ac.lowest_strouhal(t, uj))
+    oa_ref = ac.oaspl(ref["p"][:m])
+    spec_ref = ac.spectrum(t, ref["p"][:m])
+    out["oaspl_lad_db"] = [float(x) for x in oa_ref]
+    finals = all(r["q"] is not None for r in runs.values()) and m == steps // every
+    sh_ref = {"axis": ac.shock_positions(ref["q"], zc, 0), "lip": ac.shock_positions(ref["q"], zc, j_lip)} if finals else None
+    if sh_ref is not None:
+        out["shocks_lad"] = {"axis": len(sh_ref["axis"]), "lip": len(sh_ref["lip"])}
     for name in variants:
         r = runs[name]
-        if not r["finite"]:
-            out[name] = {"finite": False, "error": r.get("error", "non-finite state")}
-            continue
-        d = np.abs(r["oaspl"] - ref["oaspl"])
-        out[name] = {"finite": True, "oaspl_err_db_max": float(d.max()), "oaspl_err_db_mean": float(d.mean()),
-                     "spectrum_err_db": float(np.mean([ac.spectrum_error_db(r["spec"][1][:, k], ref["spec"][1][:, k])
-                                                      for k in range(len(pts))])),
-                     "shock_shift_axis_D": _finite(ac.shock_shift(r["shocks"]["axis"], ref["shocks"]["axis"])),
-                     "shock_shift_lip_D": _finite(ac.shock_shift(r["shocks"]["lip"], ref["shocks"]["lip"])),
-                     "shocks_axis": len(r["shocks"]["axis"]), "shocks_lip": len(r["shocks"]["lip"]),
-                     "fallback_fraction": r["fallback"]}
+        pr = r["p"][:m]
+        d = np.abs(ac.oaspl(pr) - oa_ref)
+        spec = ac.spectrum(r["t"][:m], pr)
+        o = {"oaspl_err_db_max": float(d.max()), "oaspl_err_db_mean": float(d.mean()),
+             "spectrum_err_db": float(np.mean([ac.spectrum_error_db(spec[1][:, k], spec_ref[1][:, k])
+                                              for k in range(len(pts))])),
+             "fallback_fraction": r["fallback"]}
+        if sh_ref is not None:
+            sh = {"axis": ac.shock_positions(r["q"], zc, 0), "lip": ac.shock_positions(r["q"], zc, j_lip)}
+            o.update(shock_shift_axis_D=_finite(ac.shock_shift(sh["axis"], sh_ref["axis"])),
+                     shock_shift_lip_D=_finite(ac.shock_shift(sh["lip"], sh_ref["lip"])),
+                     shocks_axis=len(sh["axis"]), shocks_lip=len(sh["lip"]))
+        out[name] = o
     return out
 
 

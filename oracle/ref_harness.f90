@@ -17,6 +17,11 @@
 !                    ControlForcing is a PARAMETER = 0, so the call is made here, after the
 !                    BCs instead of before them) with Vz_Control_Vr_Feedback.dat from the
 !                    working directory -> <out>forcingrhs.bin
+!   mode = 'write' : the reference's WriteOutput of the initial condition (the output
+!                    ModSolver.f90:108 writes before the time loop) -> RocFlo-CM.########.q in
+!                    the working directory
+!   h_write = .true. with mode 'steps': WriteOutput after the last step as well (a restart
+!                    file: a later run with h_ic pointing at it is the reference's restart)
 PROGRAM ref_harness
   USE ModuleVariables
   USE ModInputOutput
@@ -39,11 +44,13 @@ PROGRAM ref_harness
   CHARACTER(LEN=16) :: h_mode
   DOUBLE PRECISION :: h_re, h_pr, h_cfl, h_cmu, h_cbeta, h_ckappa
   INTEGER :: h_shock, h_nsteps, u, s
+  LOGICAL :: h_write
   DOUBLE PRECISION, ALLOCATABLE :: dts(:), scratch(:,:,:)
   NAMELIST /harness/ h_grid, h_ic, h_target, h_bc, h_re, h_pr, h_cfl, h_shock, h_mode, h_nsteps, &
-                     h_out, h_cmu, h_cbeta, h_ckappa
+                     h_out, h_cmu, h_cbeta, h_ckappa, h_write
 
   h_re = 1000.0D0; h_pr = 0.72D0; h_cfl = 0.5D0; h_shock = 0; h_nsteps = 1; h_mode = 'rhs'
+  h_write = .FALSE.
   h_cmu = 0.0D0; h_cbeta = 0.0D0; h_ckappa = 0.0D0
   CALL get_command_argument(1, nmlfile)
   OPEN(NEWUNIT=u, FILE=TRIM(nmlfile), STATUS='old')
@@ -111,6 +118,8 @@ PROGRAM ref_harness
     iteration = iteration_start + 1
     CALL ControlForcingTerms
     CALL dump_state(rhs, TRIM(h_out)//'forcingrhs.bin')
+  ELSEIF (TRIM(h_mode) == 'write') THEN
+    CALL WriteOutput
   ELSEIF (TRIM(h_mode) == 'steps') THEN
     ALLOCATE(dts(h_nsteps))
     CALL ComputeTimeStep
@@ -121,6 +130,7 @@ PROGRAM ref_harness
       CALL RK4
     ENDDO
     CALL dump_state(pv, TRIM(h_out)//'state.bin')
+    IF (h_write) CALL WriteOutput
     OPEN(NEWUNIT=u, FILE=TRIM(h_out)//'dt.bin', ACCESS='stream', FORM='unformatted', STATUS='replace')
     WRITE(u) dts
     WRITE(u) time

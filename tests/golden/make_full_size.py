@@ -11,8 +11,17 @@ and 20-100 step trajectories, with and without LAD).  Stored per method ("plain"
   sub{N}         state[::16, ::16, :] after N steps (N = 10, 1000)
   edges{N}       rows j = 1, Nr and columns i = 1, Nz of the state after N steps
   absmax{N}      max |state| per variable after N steps
+  rowsum{N}      sum over i of the state, per row j and variable (nr, 5), after N steps
+  colsum{N}      sum over j of the state, per column i and variable (nz, 5): with rowsum,
+                 every point of the grid enters two of the compared sums
 
-    python tests/golden/make_full_size.py [plain|lad ...] [--threads T]
+Cases: "jet4400_plain", "jet4400_lad" (the seeded 4400 x 640 Mach 1.5 jet, 1000 steps) and
+"nozzle4400_plain", "nozzle4400_lad" (the same grid with the lines of the reference's
+bc_4400x640_nozzle.dat, jetcase.NOZZLE_BC_4400x640, and the points inside the nozzle lip
+blanked; 100 steps).
+
+    python tests/golden/make_full_size.py [jet4400_plain|jet4400_lad|nozzle4400_plain|nozzle4400_lad ...]
+                                          [--threads T]
 """
 from __future__ import annotations
 
@@ -32,12 +41,22 @@ from axisymmetricjet_amd import jetcase as jc  # noqa: E402
 from cases import LAD_C, input_digest  # noqa: E402
 from oracle import pyoracle as po  # noqa: E402
 
-NZ, NR, SEED, NSTEPS = 4400, 640, 1234, 1000
-SNAPS = (10, 1000)
+NZ, NR, SEED = 4400, 640, 1234
+# name: (case factory, shock capturing, snapshot steps)
+CASES = {
+    "jet4400_plain": (lambda: jc.make_jet(NZ, NR, seed=SEED), 0, (10, 1000)),
+    "jet4400_lad": (lambda: jc.make_jet(NZ, NR, seed=SEED), 1, (10, 1000)),
+    "nozzle4400_plain": (lambda: nozzle_case(), 0, (10, 100)),
+    "nozzle4400_lad": (lambda: nozzle_case(), 1, (10, 100)),
+}
 
 
-def make_case():
-    return jc.make_jet(NZ, NR, seed=SEED)
+def nozzle_case():
+    """The 4400 x 640 grid with bc_4400x640_nozzle.dat's lines: inner wall j = 150, outer wall
+    j = 169, lip face i = 597 (1-based)."""
+    case = jc.make_nozzle(NZ, NR, i_lip=597, j_in=150, j_out=169, seed=SEED)
+    assert [tuple(l) for l in case.bc_lines] == [tuple(l) for l in jc.NOZZLE_BC_4400x640]
+    return case
 
 
 def edges(q):
@@ -48,28 +67,31 @@ def sample(q):
     return np.ascontiguousarray(q[::16, ::16, :])
 
 
-def make(method, threads):
-    case = make_case()
-    shock = 1 if method == "lad" else 0
+def make(name, threads):
+    factory, shock, snaps = CASES[name]
+    case = factory()
+    nsteps = snaps[-1]
     c = LAD_C if shock else (0.0, 0.0, 0.0)
     o = po.Oracle(case, shock=shock, c_mu=c[0], c_beta=c[1], c_kappa=c[2], threads=threads)
     out = {"input_sha256": np.array(input_digest(case)), "shock": np.array(shock), "lad_c": np.array(c),
-           "nsteps": np.array(NSTEPS)}
+           "nsteps": np.array(nsteps), "snaps": np.array(snaps)}
     dts, done, t0 = [], 0, time.time()
-    for snap in SNAPS:
+    for snap in snaps:
         while done < snap:
             n = min(50, snap - done)
             dts.append(o.run(n))
             done += n
-            print(f"[{method}] {done}/{NSTEPS} steps, {time.time() - t0:.0f} s", flush=True)
+            print(f"[{name}] {done}/{nsteps} steps, {time.time() - t0:.0f} s", flush=True)
         q = o.state()
         out[f"sub{snap}"] = sample(q)
         out[f"edges{snap}"] = edges(q)
         out[f"absmax{snap}"] = np.abs(q).max(axis=(0, 1))
+        out[f"rowsum{snap}"] = q.sum(axis=0)
+        out[f"colsum{snap}"] = q.sum(axis=1)
     out["dts"] = np.concatenate(dts)
     out["time"] = np.array(o.time())
     o.close()
-    path = os.path.join(HERE, f"jet4400_{method}.npz")
+    path = os.path.join(HERE, f"{name}.npz")
     np.savez_compressed(path, **out)
     print(f"wrote {path}: dt[0]={out['dts'][0]:.6e} dt[-1]={out['dts'][-1]:.6e} time={float(out['time']):.6e}",
           flush=True)
@@ -77,11 +99,11 @@ def make(method, threads):
 
 def main(argv):
     ap = argparse.ArgumentParser()
-    ap.add_argument("methods", nargs="*", default=["plain", "lad"])
+    ap.add_argument("names", nargs="*", default=list(CASES))
     ap.add_argument("--threads", type=int, default=0)
     args = ap.parse_args(argv)
     po.build()
-    for m in args.methods:
+    for m in args.names:
         make(m, args.threads)
     return 0
 

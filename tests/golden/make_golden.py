@@ -30,16 +30,54 @@ HARNESS = os.environ.get("REF_HARNESS") or os.path.join(ROOT, "oracle", "_ref", 
 from cases import LAD_C, cases, feedback_forcing, input_digest  # noqa: E402
 
 
-def run_harness(tmp, paths, case, mode, shock, nsteps=1, out="o_"):
-    nml = os.path.join(tmp, f"{mode}.nml")
+def run_harness(tmp, paths, case, mode, shock, nsteps=1, out="o_", write=False, tag=""):
+    nml = os.path.join(tmp, f"{mode}{tag}.nml")
     with open(nml, "w") as f:
         f.write("&harness\n")
         f.write(f" h_grid='{paths['grid']}', h_ic='{paths['ic']}', h_target='{paths['target']}',\n")
         f.write(f" h_bc='{paths['bc']}', h_re={case.re!r}d0, h_pr={case.pr!r}d0, h_cfl=0.5d0,\n")
         f.write(f" h_shock={shock}, h_mode='{mode}', h_nsteps={nsteps}, h_out='{os.path.join(tmp, out)}',\n")
-        f.write(f" h_cmu={LAD_C[0]!r}d0, h_cbeta={LAD_C[1]!r}d0, h_ckappa={LAD_C[2]!r}d0\n/\n")
-    with open(os.path.join(tmp, f"{mode}.log"), "w") as log:
+        f.write(f" h_cmu={LAD_C[0]!r}d0, h_cbeta={LAD_C[1]!r}d0, h_ckappa={LAD_C[2]!r}d0,\n")
+        f.write(f" h_write={'.true.' if write else '.false.'}\n/\n")
+    with open(os.path.join(tmp, f"{mode}{tag}.log"), "w") as log:
         subprocess.check_call([HARNESS, nml], cwd=tmp, stdout=log, stderr=subprocess.STDOUT)
+
+
+PLOT3D_TAU = (0.0, 1.5, 2.5, 0.0)  # tau(2:3) of the input pass through to every output
+PLOT3D_STEPS = 20
+
+
+def make_plot3d():
+    """plot3d88.npz: the files the reference's WriteOutput writes (ModInputOutput.f90:201-263,
+    Write_Soln ModPLOT3D_IO.f90:539-599) for the 88 x 32 jet -- its initial condition, and the
+    state after PLOT3D_STEPS steps -- as raw bytes, and the reference's restart from that second
+    file (a fresh run with it as the initial condition, ModSolver.f90:103-106): the state, time
+    steps and time after PLOT3D_STEPS more steps, and that run's output file."""
+    case = jc.make_jet(88, 32, seed=7)
+    nz, nr = case.nz, case.nr
+    out = {"input_sha256": np.array(input_digest(case)), "tau_in": np.array(PLOT3D_TAU),
+           "nsteps": np.array(PLOT3D_STEPS)}
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        paths = jc.write_case(os.path.join(tmp, "c_"), case)
+        jc.write_soln(paths["ic"], case.q0, tau=PLOT3D_TAU)
+        raw = lambda fn: np.frombuffer(open(os.path.join(tmp, fn), "rb").read(), np.uint8)  # noqa: E731
+        rd = lambda fn: np.fromfile(os.path.join(tmp, fn), "<f8")  # noqa: E731
+        run_harness(tmp, paths, case, "write", 0)
+        out["ic_file"] = raw("RocFlo-CM.00000000.q")
+        run_harness(tmp, paths, case, "steps", 0, nsteps=PLOT3D_STEPS, write=True, out="a_")
+        name1 = f"RocFlo-CM.{PLOT3D_STEPS:08d}.q"
+        out["out1_file"] = raw(name1)
+        d = rd("a_dt.bin")
+        out["dts1"], out["time1"] = d[:PLOT3D_STEPS], d[PLOT3D_STEPS]
+        restart = dict(paths, ic=os.path.join(tmp, name1))
+        run_harness(tmp, restart, case, "steps", 0, nsteps=PLOT3D_STEPS, write=True, out="b_", tag="2")
+        out["out2_file"] = raw(f"RocFlo-CM.{2 * PLOT3D_STEPS:08d}.q")
+        d = rd("b_dt.bin")
+        out["dts2"], out["time2"] = d[:PLOT3D_STEPS], d[PLOT3D_STEPS]
+        out["state2"] = rd("b_state.bin").reshape((nz, nr, 5), order="F")
+    np.savez_compressed(os.path.join(HERE, "plot3d88.npz"), **out)
+    print(f"plot3d88: wrote plot3d88.npz  files of {len(out['ic_file'])}, {len(out['out1_file'])}, "
+          f"{len(out['out2_file'])} bytes; restart time {float(out['time2']):.9e}")
 
 
 def make(name):
@@ -47,7 +85,9 @@ def make(name):
     case = factory()
     nz, nr = case.nz, case.nr
     out = {"input_sha256": np.array(input_digest(case)), "shock": np.array(shock),
-           "nsteps": np.array(nsteps), "lad_c": np.array(LAD_C)}
+           "nsteps": np.array(nsteps),
+           # the LAD constants in effect: none with shock capturing off (ShockCapturing = 0)
+           "lad_c": np.array(LAD_C if shock else (0.0, 0.0, 0.0))}
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
         paths = jc.write_case(os.path.join(tmp, "c_"), case)
         run_harness(tmp, paths, case, "rhs", shock)
@@ -88,9 +128,9 @@ def make(name):
 def main(argv):
     if not os.path.exists(HARNESS):
         subprocess.check_call(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")])
-    names = argv or list(cases())
+    names = argv or list(cases()) + ["plot3d88"]
     for n in names:
-        make(n)
+        make_plot3d() if n == "plot3d88" else make(n)
 
 
 if __name__ == "__main__":

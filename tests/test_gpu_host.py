@@ -109,3 +109,57 @@ def test_fortran_solver_with_learned_lad(tmp_path):
     plain = capi.Context(case, shock_capturing=1, C_mu=c[0], C_beta=c[1], C_kappa=c[2])
     plain.run(12)
     assert not np.array_equal(q, plain.state())
+
+
+def run_solver(tmp_path, paths, case, ic, nsteps, noutput, tag):
+    nml = tmp_path / f"{tag}.nml"
+    nml.write_text(
+        "&ajet_input\n"
+        f" GRID_FILENAME='{paths['grid']}', INITIAL_CONDITION='{ic}',\n"
+        f" TARGET_FILE='{paths['target']}', bc_file='{paths['bc']}', out_prefix='{tmp_path}/',\n"
+        f" Re={case.re!r}, Pr={case.pr!r}, CFL=0.5, no_iterations={nsteps}, noutput={noutput}\n/\n")
+    print(f"running ajet_solver ({tag}): {nsteps} steps from {os.path.basename(ic)}", flush=True)
+    out = subprocess.run([SOLVER, str(nml)], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = re.findall(r"Iteration=(\d+)\s+dt=\s*(\S+)\s+time=\s*(\S+)", out.stdout)
+    return [int(k) for k, _, _ in lines], np.array([float(d) for _, d, _ in lines])
+
+
+def test_fortran_solver_output_and_restart_match_the_reference(tmp_path):
+    """The files the host writes are the reference's: its RocFlo-CM.00000000.q is byte for byte
+    the one the reference's WriteOutput writes for the same initial condition, and after 20 steps
+    its RocFlo-CM.00000020.q has the reference's headers (ngrid, ND, tau(1) = 20, tau(2:3) passed
+    through, tau(4) = time) and a state within 1e-9.  A second run from that file is a restart
+    (ModSolver.f90:103-106: iteration_start = tau(1), time = tau(4), and the stale temperature of
+    the time step back at 0): 20 more steps match the reference's own restart from its own file
+    (tests/golden/plot3d88.npz), state within 1e-9, time within 1e-12."""
+    g = dict(np.load(os.path.join(ROOT, "tests", "golden", "plot3d88.npz")))
+    case = jc.make_jet(88, 32, seed=7)
+    assert str(g["input_sha256"]) == gc.input_digest(case)
+    n = int(g["nsteps"])
+    paths = jc.write_case(str(tmp_path / "c_"), case)
+    jc.write_soln(paths["ic"], case.q0, tau=tuple(g["tau_in"]))
+    its, dts = run_solver(tmp_path, paths, case, paths["ic"], n, n, "first")
+    assert its == list(range(1, n + 1))
+    np.testing.assert_allclose(dts, g["dts1"], rtol=1e-8)          # E17.9 print precision
+    assert (tmp_path / "RocFlo-CM.00000000.q").read_bytes() == g["ic_file"].tobytes()
+    f1 = (tmp_path / f"RocFlo-CM.{n:08d}.q").read_bytes()
+    ref1 = g["out1_file"].tobytes()
+    head = 12 + 20 + 4                      # ngrid and ND records, the tau record's marker
+    assert len(f1) == len(ref1) and f1[:head] == ref1[:head] and f1[-4:] == ref1[-4:]
+    q1, tau1 = jc.read_soln(str(tmp_path / f"RocFlo-CM.{n:08d}.q"))
+    rq1, rtau1 = jc.read_soln_bytes(ref1)
+    assert tau1[:3] == rtau1[:3] and abs(tau1[3] - rtau1[3]) <= 1e-12 * rtau1[3]
+    e1 = float(np.abs(q1 - rq1).max())
+    # the restart from the host's own file
+    its2, dts2 = run_solver(tmp_path, paths, case, str(tmp_path / f"RocFlo-CM.{n:08d}.q"), n, n, "restart")
+    assert its2 == list(range(n + 1, 2 * n + 1))
+    np.testing.assert_allclose(dts2, g["dts2"], rtol=1e-8)
+    q2, tau2 = jc.read_soln(str(tmp_path / f"RocFlo-CM.{2 * n:08d}.q"))
+    rq2, rtau2 = jc.read_soln_bytes(g["out2_file"].tobytes())
+    assert tau2[:3] == rtau2[:3] and tau2[0] == 2.0 * n
+    assert abs(tau2[3] - float(g["time2"])) <= 1e-12 * float(g["time2"])
+    assert np.array_equal(rq2, g["state2"])
+    e2 = float(np.abs(q2 - g["state2"]).max())
+    print(f"fortran host: step {n} err {e1:.3e}, after the restart (step {2 * n}) err {e2:.3e}", flush=True)
+    assert e1 <= 1e-9 and e2 <= 1e-9

@@ -35,13 +35,13 @@ def test_trained_coefficients_beat_untrained_and_zero(case):
 
 
 def test_learned_mode_acoustics_beat_the_method_without_the_network(case):
-    res = ll.score(case, ll.load(), sensor_threshold=0.1, steps=1200, every=4, refresh=8)
-    assert res["reference_finite"]
+    """From the jet advanced 400 steps under the reference LAD, 1200 more steps under each mode."""
+    res = ll.score(case, ll.load(), sensor_threshold=0.1, steps=1200, every=4, refresh=8, warm=400)
+    assert res["all_finite"], res["finite_steps"]
     lr, zr = res["learned"], res["zero"]
-    print(f"OASPL error vs reference LAD, 1200 steps: learned max {lr['oaspl_err_db_max']:.3f} dB "
+    print(f"OASPL error vs reference LAD, 1200 steps after 400: learned max {lr['oaspl_err_db_max']:.3f} dB "
           f"mean {lr['oaspl_err_db_mean']:.3f}; without the network max {zr['oaspl_err_db_max']:.3f} "
-          f"mean {zr['oaspl_err_db_mean']:.3f}", flush=True)
-    assert lr["finite"] and zr["finite"]
+          f"mean {zr['oaspl_err_db_mean']:.3f}; record {res['record_time_D_per_c']:.3f} D/c", flush=True)
     assert lr["oaspl_err_db_max"] <= 0.5
     assert lr["oaspl_err_db_mean"] < zr["oaspl_err_db_mean"]
     assert lr["spectrum_err_db"] < zr["spectrum_err_db"]
