@@ -17,7 +17,7 @@ Prints one JSON line.
              and writes q^{n+1}), divided by the summed average durations of the four stages
              measured with HIP events on the library's stream.  traffic is the HBM bytes per
              step measured with rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE) for the same
-             command, kept in MEASURED_TRAFFIC
+             command, kept in profiles/traffic.json (null when the sources changed since)
   cpu_baseline  the C restatement (oracle) of the same step on the host cores, OpenMP over
              grid lines, on a bounded sample of steps of the same grid
   learned    the same jet with the learned LAD coefficient (ajet_set_learned_lad, the weights in
@@ -44,14 +44,76 @@ HBM_PEAK_GBPS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0   # dense MFMA
 MLP_FLOP_PER_POINT = 2 * (104 * 128 + 2 * 128 * 128 + 128 * 2)
 
-# HBM bytes per grid-point update of a whole step, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE over the same workload (FETCH scaled by 2.0 and WRITE by 1.0, the factors a 1 GiB
-# device copy reads on the same counters); None where not measured.
-MEASURED_TRAFFIC = {"plain": 723.9, "lad": 1460.8, "learned": 891.0}  # profiles/step_jet4400x640_{plain,lad}.md, learned_lad_jet4400x640.md
+# HBM bytes per grid-point update of a whole step, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE over the same workload (bench.py --traffic-from), kept in profiles/traffic.json
+# with a digest of the library sources they were measured on; a digest that no longer matches
+# the sources gives traffic = null rather than a stale number.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+CALIB_MIB = 1024
+
+
+def sources_digest():
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "axisymmetricjet_amd", "csrc")
+    for f in sorted(os.listdir(d)) + ["../build.py"]:
+        with open(os.path.join(d, f), "rb") as fh:
+            h.update(f.encode() + b"\0" + fh.read())
+    return h.hexdigest()
+
+
+def measured_traffic(method):
+    """HBM bytes per grid-point update of `method` from profiles/traffic.json, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            doc = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if doc.get("sources_sha256") != sources_digest():
+        return None
+    return doc.get("bytes_per_update", {}).get(method)
+
+
+def traffic_from(root, methods, points):
+    """profiles/traffic.json from the rocprofv3 counter runs under root: for each method,
+    root/pmc_<method>_FETCH_SIZE and root/pmc_<method>_WRITE_SIZE hold the counter CSVs of
+    `bench.py --method <method> ... --calib-copy-mib 1024`.  Each counter is scaled by what it
+    reads for that run's 1 GiB device copy (the dispatch whose name holds "copy" with the largest
+    count), and the per-step bytes are the step's kernels summed over the dispatches of the run
+    divided by its stage-4 launches (one per step)."""
+    import csv
+    out = {"sources_sha256": sources_digest(), "points": points, "bytes_per_update": {}, "scale": {}}
+    for m in methods:
+        per = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            path = os.path.join(root, f"pmc_{m}_{counter}", "run_counter_collection.csv")
+            rows = [r for r in csv.DictReader(open(path)) if r["Counter_Name"] == counter]
+            calib = max((r for r in rows if "copy" in r["Kernel_Name"].lower()), key=lambda r: float(r["Counter_Value"]))
+            scale = CALIB_MIB * 1024.0 / float(calib["Counter_Value"])
+            step = [r for r in rows if any(k in r["Kernel_Name"] for k in ("stage_", "lad_", "mlp_", "cfl_"))]
+            nsteps = sum(1 for r in step if "<4," in r["Kernel_Name"] and ("true" in r["Kernel_Name"] or "mixed" in r["Kernel_Name"]))
+            kib = sum(float(r["Counter_Value"]) for r in step)
+            per[counter] = kib * 1024.0 * scale / max(nsteps, 1) / points
+            out["scale"][f"{m}_{counter}"] = scale
+        out["bytes_per_update"][m] = per["FETCH_SIZE"] + per["WRITE_SIZE"]
+        out.setdefault("split", {})[m] = per
+    with open(TRAFFIC_FILE, "w") as f:
+        json.dump(out, f, indent=1)
+    return out
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(case, seconds: float, threads: int, lad_c):
@@ -90,7 +152,20 @@ def main(argv=None):
     ap.add_argument("--learned-refresh", type=int, default=8, help="the network runs every this many steps")
     ap.add_argument("--sensor-threshold", type=float, default=0.1)
     ap.add_argument("--ramp-min", type=float, default=1e-8)
-    ap.add_argument("--score-steps", type=int, default=5000, help="steps of the learned-LAD scoring runs (0: skip)")
+    ap.add_argument("--score-steps", type=int, default=4000,
+                    help="steps of the learned-LAD scoring runs of the ideally expanded jet (0: skip)")
+    ap.add_argument("--score-warm", type=int, default=1500,
+                    help="steps under the reference LAD before the scoring record (the jet develops; under the "
+                         "reference's numerics the runs leave the finite range near step 5800 from the initial "
+                         "condition, so warm-up + record stay below that)")
+    ap.add_argument("--shocked-score-steps", type=int, default=1400,
+                    help="steps of the scoring runs of the under-expanded jet, npr 1.5 (0: skip; finite to ~2200 "
+                         "steps from its initial condition)")
+    ap.add_argument("--shocked-warm", type=int, default=750)
+    ap.add_argument("--combined-long-steps", type=int, default=0,
+                    help="one long run of the combined batch (the configured 10000 steps): learned and reference-LAD "
+                         "batches with probes every 8 steps, ms/step, the step each run leaves the finite range, "
+                         "OASPL difference over the record all runs share (0: skip)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "axisymmetricjet_amd", "data", "learned_lad_4400x640.npz"))
     ap.add_argument("--combined-runs", type=int, default=32,
                     help="runs of the combined form: learned-LAD jets at the bench grid with a forcing sweep (0: skip)")
@@ -99,10 +174,15 @@ def main(argv=None):
                     help="steps of the combined batch's OASPL comparison with the same batch under the reference LAD")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic-from", default=None,
+                    help="write profiles/traffic.json from the rocprofv3 --pmc runs under this folder and exit")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run one device copy of this many MiB (a known-size kernel that scales "
                          "hardware byte counters)")
     args = ap.parse_args(argv)
+    if args.traffic_from:
+        print(json.dumps(traffic_from(args.traffic_from, ("plain", "lad", "learned"), args.nz * args.nr)))
+        return 0
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it (use --gpus 1)",
               file=sys.stderr)
@@ -234,11 +314,23 @@ def main(argv=None):
                                           "flop_per_point": MLP_FLOP_PER_POINT}}
         log(f"learned: {learned['ms_per_step']:.4f} ms/step, x{learned['speedup_vs_lad'] or 0:.3f} vs LAD, "
             f"inference {mlp_ms:.4f} ms ({tflops:.1f} TFLOP/s), fallback {frac:.4f}")
-        if args.score_steps > 0:
-            t0 = time.time()
-            learned["score"] = ll.score(case, model, args.sensor_threshold, steps=args.score_steps, every=4,
-                                        ramp_min=args.ramp_min, refresh=args.learned_refresh)
-            log(f"learned: scoring runs of {args.score_steps} steps in {time.time() - t0:.1f} s")
+        if args.score_steps > 0 or args.shocked_score_steps > 0:
+            uj = 1.5 * float(np.sqrt(jc.TJ_TINF))
+            learned["score"] = {}
+            for name, sc, steps, warm, thr in (
+                    ("developed", case, args.score_steps, args.score_warm, args.sensor_threshold),
+                    ("shocked_npr1.5", jc.make_jet(nz, nr, seed=1234, npr=1.5) if args.shocked_score_steps > 0 else None,
+                     args.shocked_score_steps, args.shocked_warm, 0.05)):
+                if steps <= 0:
+                    continue
+                t0 = time.time()
+                learned["score"][name] = ll.score(sc, model, thr, steps=steps, every=4, ramp_min=args.ramp_min,
+                                                  refresh=args.learned_refresh, warm=warm, uj=uj)
+                r = learned["score"][name]
+                log(f"learned: {name} jet scored over {steps} steps after {warm} in {time.time() - t0:.1f} s: "
+                    f"record {r.get('record_time_D_per_c', 0):.3f} D/c, OASPL error learned "
+                    f"{r.get('learned', {}).get('oaspl_err_db_mean', float('nan')):.4f} dB (mean), without the network "
+                    f"{r.get('zero', {}).get('oaspl_err_db_mean', float('nan')):.4f} dB")
 
     ensemble = None
     if args.ensemble_runs > 0 and args.ensemble_steps > 0:
@@ -338,6 +430,52 @@ def main(argv=None):
             b.close()
             for r in runs:
                 r.close()
+        if args.combined_long_steps > 0:
+            # the configured run length, once: both batches with probes every 8 steps
+            N, every = args.combined_long_steps, 8
+            long = {"steps": N, "probe_every": every}
+            hist_l, bad = {}, {}
+            for name in ("learned", "lad"):
+                t0 = time.time()
+                runs, b = combined_batch(name == "learned")
+                bstream = torch.cuda.ExternalStream(runs[0].stream())
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(bstream)
+                ps, first_bad = [], [None] * len(runs)
+                for k in range(N // every):
+                    b.run(every)
+                    vs = [r.probe() for r in runs]
+                    row = [v[:, 0] * v[:, 4] for v in vs]
+                    for ri, p in enumerate(row):
+                        if first_bad[ri] is None and not np.all(np.isfinite(p)):
+                            first_bad[ri] = (k + 1) * every
+                    ps.append(row)
+                    if (k + 1) * every % 1000 == 0:
+                        log(f"combined long ({name}): step {(k + 1) * every}/{N}, "
+                            f"{sum(x is not None for x in first_bad)} runs non-finite, {time.time() - t0:.0f} s")
+                e1.record(bstream)
+                torch.cuda.synchronize()
+                lms = e0.elapsed_time(e1)
+                long[name] = {"ms_per_step": lms / N, "value": cpts * N / (lms / 1e3),
+                              "runs_nonfinite": sum(x is not None for x in first_bad),
+                              "first_nonfinite_step": min((x for x in first_bad if x is not None), default=None)}
+                hist_l[name], bad[name] = np.array(ps), first_bad
+                log(f"combined long ({name}): {lms / N:.3f} ms/step with probes, "
+                    f"{long[name]['runs_nonfinite']} of {len(runs)} runs left the finite range "
+                    f"(first at step {long[name]['first_nonfinite_step']})")
+                b.close()
+                for r in runs:
+                    r.close()
+            # OASPL per run over the samples both of its runs keep finite
+            errs = []
+            for ri in range(len(specs)):
+                m = min(x // every - 1 if x is not None else N // every for x in (bad["learned"][ri], bad["lad"][ri]))
+                if m >= 16:
+                    errs.append(np.abs(ac.oaspl(hist_l["learned"][:m, ri, :]) - ac.oaspl(hist_l["lad"][:m, ri, :])))
+            long["oaspl_err_db_max"] = float(np.max(errs)) if errs else None
+            long["oaspl_err_db_mean"] = float(np.mean(errs)) if errs else None
+            long["runs_scored"] = len(errs)
+            combined["long"] = long
         combined["speedup_vs_lad"] = combined["lad"]["ms_per_step"] / combined["learned"]["ms_per_step"]
         if hist:
             oa = {k: np.stack([ac.oaspl(v[:, r, :]) for r in range(v.shape[1])]) for k, v in hist.items()}
@@ -348,12 +486,17 @@ def main(argv=None):
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: oracle, {threads} threads, ~{args.cpu_seconds:.0f} s")
+        # every core this job may use: the GPU box gives a one-GPU job a share of its host cores
+        # (OMP_NUM_THREADS), whatever os.cpu_count() shows there
+        host_cpus = os.cpu_count() or 1
+        threads = min(host_cpus, int(os.environ.get("OMP_NUM_THREADS") or host_cpus))
+        log(f"cpu baseline: oracle, {threads} threads of {host_cpus} ({cpu_model()}), ~{args.cpu_seconds:.0f} s")
         v, n, el = cpu_baseline(case, args.cpu_seconds, threads, jc.LAD_C if args.method != "plain" else None)
         cpu = {"value": v, "unit": "grid-point updates/s", "cores": threads, "kind": "port",
+               "cpu_model": cpu_model(), "host_cpus": host_cpus,
                "sample": f"{n} RK4 steps ({'lad' if args.method == 'learned' else args.method}) of the same {nz}x{nr} jet (after 1 warm-up step), "
-                         f"{el:.1f} s, C restatement at -O2 without FMA, OpenMP over grid lines"}
+                         f"{el:.1f} s, C restatement at -O2 without FMA, OpenMP over grid lines, {threads} threads "
+                         f"(the job's share of the {host_cpus} host CPUs)"}
         log(f"cpu baseline {v:.4e} updates/s")
 
     out = {
@@ -377,7 +520,8 @@ def main(argv=None):
                    "nz": nz, "nr": nr, "points": npts, "method": args.method},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS,
-                     "traffic": MEASURED_TRAFFIC[args.method] * npts if MEASURED_TRAFFIC[args.method] and (nz, nr) == (4400, 640) else None,
+                     "traffic": (measured_traffic(args.method) * npts
+                                 if measured_traffic(args.method) and (nz, nr) == (4400, 640) else None),
                      "kernel": "the four RK stages of a step, each the artificial-diffusivity launches (lad) + the stage kernel",
                      "stage_ms": stage_ms, "bytes_per_update": BYTES_PER_UPDATE},
         "other_methods": others,
