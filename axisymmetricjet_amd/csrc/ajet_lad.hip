@@ -193,9 +193,32 @@ __device__ __forceinline__ void source_tile(const LadArgs& a, int t, SourceLds& 
     // not all at once: the kernel's 72 VGPRs hold three loads in flight without spilling)
     constexpr int SW = 3;
     constexpr int NG = ((LWZ * LWR + 255) / 256 + SW - 1) / SW;
+    if constexpr (INT) {
+      // interior stencils reach 4 points from the tile, not LH = 7: only that part of the window
+      // is read (its points lie inside the grid; indices are clamped for tiles narrower than the
+      // window, whose extra points go unused)
+      constexpr int IB = LH - 4, IWZ = LWZ - 2 * IB, IWR = LWR - 2 * IB, NI = IWZ * IWR;
+      constexpr int NGI = ((NI + 255) / 256 + SW - 1) / SW;
+#pragma unroll 1
+      for (int gq = 0; gq < NGI; ++gq) {
+        double v[SW];
+        int dst[SW];
+#pragma unroll
+        for (int u = 0; u < SW; ++u) {
+          const int k = (int)threadIdx.x + 256 * (SW * gq + u);
+          const int iz = IB + k % IWZ, ir = IB + k / IWZ;
+          const int i = min(i0 - LH + iz, nz - 1), j = min(j0 - LH + ir, nr - 1);
+          dst[u] = k < NI ? ir * LWZ + iz : -1;
+          v[u] = src[k < NI ? (long long)i + (long long)pitch * j : (long long)i0 + (long long)pitch * j0];
+        }
+#pragma unroll
+        for (int u = 0; u < SW; ++u)
+          if (dst[u] >= 0) fB[dst[u]] = v[u];
+      }
+    }
     int wz = wz0, wr = wr0;
 #pragma unroll 1
-    for (int gq = 0; gq < NG; ++gq) {
+    for (int gq = 0; gq < (INT ? 0 : NG); ++gq) {
       double v[SW];
       bool vin[SW];
 #pragma unroll
@@ -319,6 +342,16 @@ __device__ __forceinline__ void zwalk_advance(ZWalk& st, double w, double xn) {
   st.X[8] = xn;
 }
 
+// The interior filter in fused form for the chunks whose points all take it: the symmetric
+// weights pair the values either side, and the last filtered value y1 enters last, so a step
+// waits on one add and one fused multiply-add of the step before it (the reference's
+// left-to-right sum makes that six dependent operations).  It rounds differently from
+// tg_interior by a few ulp of the filtered values.
+__device__ __forceinline__ double tg_interior_fast(const ZWalk& st) {
+  const double t = fma(TG0, st.y4 + st.X[4], fma(TG1, st.y3 + st.X[3], fma(TG2, st.y2 + st.X[2], TG4 * st.X[0])));
+  return fma(TG3, st.y1 + st.X[1], t);
+}
+
 // 16 steps of a chunk whose points all take the interior filter.  Walk chunk k fills ring
 // chunk k + 1, slots 16 ((k + 1) mod 4) .., so KM = (k + 1) mod 4 fixes every slot at compile
 // time.
@@ -328,7 +361,7 @@ __device__ __forceinline__ void zwalk_interior(ZWalk& st, double (*ring)[ZLINES 
   for (int t = 0; t < ZCH; ++t) {
     const int sl = (ZCH * KM + t) & (ZRING - 1);
     const double xn = ring[(sl + 9) & (ZRING - 1)][lane];
-    const double w = tg_interior(st);
+    const double w = tg_interior_fast(st);
     ring[sl][lane] = w;
     zwalk_advance(st, w, xn);
   }
@@ -478,8 +511,8 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a) {
       const uint32_t rr = code[l][sl] >> 4;
       double v = 0.0;
       if (rr == ROW_INTERIOR) {
-        v = (((((((TG0 * w[0] + TG1 * w[1]) + TG2 * w[2]) + TG3 * w[3]) + TG4 * w[4]) + TG3 * w[5]) + TG2 * w[6]) +
-             TG1 * w[7]) + TG0 * w[8];
+        // symmetric weights paired, fused (a few ulp from the reference's left-to-right sum)
+        v = fma(TG0, w[0] + w[8], fma(TG1, w[1] + w[7], fma(TG2, w[2] + w[6], fma(TG3, w[3] + w[5], TG4 * w[4]))));
       } else if (rr != ROW_BLANK) {
         const int base = rr < 4 ? -(int)rr : -(int)rr - 2;
 #pragma unroll
