@@ -91,7 +91,7 @@ def test_tuned_tile_is_a_candidate_and_reaches_the_build():
     with open(autotune.TUNE) as f:
         doc = json.load(f)
     assert tuple(doc["tile"]) in autotune.CANDIDATES
-    assert doc["identical_results"] is True
+    assert doc["states_agree_within"] <= autotune.STATE_TOL
     tz, tr, th = doc["tile"]
     assert build.tile_defines() == [f"-DAJET_TZ={tz}", f"-DAJET_TR={tr}", f"-DAJET_THREADS={th}"]
     assert build.tile_defines((64, 8, 256)) == ["-DAJET_TZ=64", "-DAJET_TR=8", "-DAJET_THREADS=256"]
