@@ -10,6 +10,8 @@
 // Build with -ffp-contract=off.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ajet_kernels.h"
 #include "ajet_stencil.h"
 
@@ -307,18 +309,23 @@ __global__ __launch_bounds__(256, 7) void lad_source_kernel(LadArgs a) {
 // from i = 1.  The segment length is chosen per grid (lad_zfilter_segment).
 //
 // One wave walks 64 lines (one lane each) through its segment.  The columns stream through an
-// LDS ring of ZRING columns in chunks of ZCH: while the lanes filter chunk k, the loads of
-// chunk k + 4 are in flight and chunk k + 3 waits in registers, and each finished chunk is
-// written back with 128-byte rows.  The walk keeps the last four filtered values and the next five inputs in
-// registers (and reads the ring 9 points ahead), so an interior step waits only on its own
-// arithmetic; closure rows read their 11
-// points from the ring (up to 10 behind or 10 ahead).  While walk chunk k runs, the ring holds
-// ring chunks k .. k + 2 (columns ib - 16 + 16 k .. ib + 31 + 16 k), which covers every read of
-// its steps i = ib + 16 k .. ib + 16 k + 15 (columns i - 10 .. i + 10).
+// LDS ring of ZRING columns in chunks of ZCH; a second wave takes the r filter beside the walk.
+// The walk keeps the last four filtered values and the next nine inputs in registers (and
+// reads the ring 9 points ahead), so an interior step waits only on its own arithmetic; closure rows read their 11 points from the ring (up to 10 behind or 10 ahead).
+// While walk chunk k runs, the ring holds ring chunks k .. k + 2 (columns ib - 16 + 16 k ..
+// ib + 31 + 16 k), which covers every read of its steps i = ib + 16 k .. ib + 16 k + 15
+// (columns i - 10 .. i + 10).
 constexpr int ZWARM = 48;
 constexpr int ZLINES = LAD_LINES;
 constexpr int ZCH = 16;
 constexpr int ZRING = 64;
+// chunks in flight in the walker's registers: a chunk's loads are issued ZPRE iterations
+// before it enters the ring (one measured fastest: the walk of a chunk covers the loads, and a
+// second register set costs the walker 40 VGPRs)
+#ifndef AJET_ZPRE
+#define AJET_ZPRE 1
+#endif
+constexpr int ZPRE = AJET_ZPRE;
 
 
 // Walk state of one lane: the last four filtered values and the next nine unfiltered ones.
@@ -367,15 +374,26 @@ __device__ __forceinline__ void zwalk_interior(ZWalk& st, double (*ring)[ZLINES 
   }
 }
 
-// The fused z and r filters.  Block (segment, line group, field): one wave walks the group's 64
-// r-lines wb..wb+63 through the z recursion, then, chunk by chunk, takes the r filter of the
-// group's output lines [out_lo, out_hi) from the ring (PerformTGfilter along eta,
-// ModShockCapturing.f90:64, 70, 76), scales it by C Re (C_kappa Re Pr) and the ramp
-// (ModShockCapturing.f90:66-92) and writes mu*, beta* or kappa*.  Blanked points are left out
-// by the filter and keep zero.  Ring codes: z row in the low nibble, r row in the high one.
-__global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a) {
+// The fused z and r filters.  Block (segment, line group, field), two waves: the walker (wave 0)
+// takes the group's 64 r-lines wb..wb+63 through the z recursion, one lane a line, and moves the
+// chunks from memory into the ring; the writer (wave 1) takes the r filter of the chunk walked
+// before (PerformTGfilter along eta, ModShockCapturing.f90:64, 70, 76) over the group's output
+// lines [out_lo, out_hi), scales it by C Re (C_kappa Re Pr) and the ramp
+// (ModShockCapturing.f90:66-92) and writes mu*, beta* or kappa*.  All loads are the walker's and
+// all stores the writer's, so neither waits on the other's memory traffic.  One barrier per
+// chunk: in iteration k the walker walks chunk k (ring chunk k + 1, reading ring chunks k ..
+// k + 2) and then overwrites ring chunk k - 1 with ring chunk k + 3, whose loads it issued
+// ZPRE iterations before; the writer reads ring chunk k (walk chunk k - 1).  Blanked points
+// are left out by the filter and keep zero.  Ring codes: z row in the low nibble, r row in the
+// high one.
+__global__ __launch_bounds__(2 * ZLINES, 2) void lad_zrfilter_kernel(LadArgs a) {
   __shared__ double ring[ZRING][ZLINES + 1];
-  __shared__ uint8_t code[ZLINES][ZRING + 4];  // [line][slot]
+  // row codes, one byte per [line][slot], four slots a word (written as words, read as bytes)
+  __shared__ uint32_t code_w[ZLINES][(ZRING + 4) / 4];
+  __shared__ double ramp_s[ZRING / ZCH][ZCH];  // the ramp of walk chunk m's columns (slot m mod 4)
+  __shared__ double tg_s[9][11];               // the TG closure rows (c_tg), read by lanes that differ in row
+  __shared__ uint32_t rint_s[ZRING / ZCH];     // ring chunk m (slot m mod 4): every point takes the interior r row
+  __shared__ unsigned long long colmask_s[ZRING];  // slot s, where rint_s is 0: lines whose r row is not interior
   const Geometry& g = a.g;
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
@@ -390,7 +408,8 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a) {
   const double* __restrict__ x = a.B + f * fs;
   double* __restrict__ out = a.out + f * fs;
   const double C = f == 0 ? a.cmu : (f == 1 ? a.cbeta : a.ckappa);
-  const int lane = threadIdx.x;
+  const bool walker = threadIdx.x < ZLINES;
+  const int lane = (int)threadIdx.x & (ZLINES - 1);
   const int c16 = lane & (ZCH - 1), l4 = lane / ZCH;  // r filter: column c16, lines quarter l4
   const int c8 = lane & 7, l8 = lane / 8;              // chunk transfers: 8 lines x 16 columns per instruction
   const int ib = max(0, S0 - ZWARM);                   // first point the walk filters
@@ -398,61 +417,94 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a) {
   // starts and the row pitch are multiples of 16, so every chunk is one 128-byte line per line
   const int P0 = ib - ZCH;
   // ring chunk m holds columns P0 + 16 m .. P0 + 16 m + 15; walk chunk k (points ib + 16 k ..)
-  // is ring chunk k + 1.  Loads go into registers, two sets alternating: the loads of chunk
-  // k + 4 are in flight while chunk k + 3 waits to be stored.  Chunks past the last column the
-  // walk reads (S1 + 9) load the last chunk it reads again instead (cache hits): loads stay
-  // unconditional, so none waits on a branch.
+  // is ring chunk k + 1.  Chunks past the last column the walk reads (S1 + 9) load the last
+  // chunk it reads again instead (cache hits): loads stay unconditional, so none waits on a
+  // branch.
   const int mlast = (S1 + 9 - P0) / ZCH;
-  // a chunk in registers: lane (c8, l8) holds columns 2 c8, 2 c8 + 1 of lines 8 r + l8
+  // a chunk in the walker's registers: lane (c8, l8) holds columns 2 c8, 2 c8 + 1 of lines
+  // 8 r + l8, lane l the row codes of line l (16 bytes), lane c16 the ramp of column c16
   struct Chunk {
     double2 v[ZLINES / 8];
-    uint32_t cz[ZLINES / 8];  // two codes, one per byte
+    uint4 cz;
+    double ramp;
   };
-  Chunk ca, cb;
   auto col = [&](int m) { return P0 + min(m, mlast) * ZCH; };  // first column ring chunk m loads
-  unsigned long long interior = 0ull;  // bit (m mod 64): every z code of ring chunk m is interior
+  auto code = [&](int l, int sl) { return (uint32_t)reinterpret_cast<const uint8_t*>(&code_w[l][0])[sl]; };
+  unsigned long long interior = 0ull;  // walker: bit (m mod 64) -- every z code of ring chunk m is interior
   // Loads read clamped points (inside the row's pitch) as they are; the store replaces the
   // points outside the grid by zero and the blank code.  No select between the two keeps a load
   // in flight across the loop's back edge.
-  auto load_chunk = [&](Chunk& c, int q) {
+  auto load_chunk = [&](Chunk& c, int m) {
+    const int q = col(m);
 #pragma unroll
     for (int r = 0; r < ZLINES / 8; ++r) {
       const int j = min(jb + 8 * r + l8, nr - 1), i = min(max(q + 2 * c8, 0), pitch - 2);
-      const long long gi = (long long)i + (long long)pitch * j;
-      c.v[r] = *reinterpret_cast<const double2*>(x + gi);
-      c.cz[r] = *reinterpret_cast<const uint16_t*>(a.codes + gi);
+      c.v[r] = *reinterpret_cast<const double2*>(x + ((long long)i + (long long)pitch * j));
     }
+    // the codes of columns q .. q + 15 (a multiple of 16, like the pitch: inside one row)
+    const int cq = min(max(q, 0), pitch - ZCH), cj = min(jb + lane, nr - 1);
+    c.cz = *reinterpret_cast<const uint4*>(a.codes + ((long long)cq + (long long)pitch * cj));
+    c.ramp = a.ramp[min(max(q + c16, 0), nz - 1)];
   };
   auto store_chunk = [&](const Chunk& c, int m) {
+    const int q = col(m);
     const int slot = (m * ZCH + 2 * c8) & (ZRING - 1);
-    const int i = col(m) + 2 * c8;
+    const int i = q + 2 * c8;
     const bool in0 = i >= 0 && i < nz, in1 = i + 1 >= 0 && i + 1 < nz;
-    constexpr uint32_t BLANK2 = ROW_BLANK | (ROW_BLANK << 4);
-    bool all = true;
 #pragma unroll
     for (int r = 0; r < ZLINES / 8; ++r) {
       const int l = 8 * r + l8;
       const bool jin = jb + l < nr;
-      const uint32_t z0 = (jin && in0) ? (c.cz[r] & 255u) : BLANK2;
-      const uint32_t z1 = (jin && in1) ? (c.cz[r] >> 8) : BLANK2;
       ring[slot][l] = (jin && in0) ? c.v[r].x : 0.0;
       ring[slot + 1][l] = (jin && in1) ? c.v[r].y : 0.0;
-      *reinterpret_cast<uint16_t*>(&code[l][slot]) = (uint16_t)(z0 | (z1 << 8));
-      all = all && (z0 & 15u) == ROW_INTERIOR && (z1 & 15u) == ROW_INTERIOR;
+    }
+    // codes: a chunk wholly outside the grid's columns (q < 0 or q >= pitch) or lines takes the
+    // blank code; the columns nz .. pitch - 1 hold it already
+    const bool cin = q >= 0 && q < pitch && jb + lane < nr;
+    constexpr uint32_t BLANK4 = 0x01010101u * (ROW_BLANK | (ROW_BLANK << 4));
+    const uint32_t w0 = cin ? c.cz.x : BLANK4, w1 = cin ? c.cz.y : BLANK4, w2 = cin ? c.cz.z : BLANK4,
+                   w3 = cin ? c.cz.w : BLANK4;
+    uint32_t* const cw = &code_w[lane][((m * ZCH) & (ZRING - 1)) / 4];
+    cw[0] = w0;
+    cw[1] = w1;
+    cw[2] = w2;
+    cw[3] = w3;
+    if (lane < ZCH) ramp_s[(m + 3) & 3][lane] = c.ramp;  // ring chunk m is walk chunk m - 1
+    // every code interior in z (low nibbles 4), and in r (high nibbles 4)
+    constexpr uint32_t ZMASK = 0x0F0F0F0Fu, ZINT = 0x01010101u * ROW_INTERIOR;
+    constexpr uint32_t RMASK = 0xF0F0F0F0u, RINT = 0x01010101u * (ROW_INTERIOR << 4);
+    const bool zin = (w0 & ZMASK) == ZINT && (w1 & ZMASK) == ZINT && (w2 & ZMASK) == ZINT && (w3 & ZMASK) == ZINT;
+    const bool rin = (w0 & RMASK) == RINT && (w1 & RMASK) == RINT && (w2 & RMASK) == RINT && (w3 & RMASK) == RINT;
+    const bool rall = __all(rin);
+    if (lane == 0) rint_s[m & 3] = rall ? 1u : 0u;
+    if (!rall) {
+      // column c of the chunk: bit l set where line l takes a closure or blanked r row
+      unsigned long long mine = 0ull;
+#pragma unroll
+      for (int cc = 0; cc < ZCH; ++cc) {
+        const uint32_t wv = cc < 4 ? w0 : (cc < 8 ? w1 : (cc < 12 ? w2 : w3));
+        const unsigned long long bm = __ballot(((wv >> (8 * (cc & 3) + 4)) & 15u) != ROW_INTERIOR);
+        mine = lane == cc ? bm : mine;
+      }
+      if (lane < ZCH) colmask_s[((m * ZCH) & (ZRING - 1)) + lane] = mine;
     }
     const unsigned long long bit = 1ull << (m & 63);
-    interior = __all(all) ? (interior | bit) : (interior & ~bit);
+    interior = __all(zin) ? (interior | bit) : (interior & ~bit);
   };
-  load_chunk(ca, col(0));
-  load_chunk(cb, col(1));
-  store_chunk(ca, 0);
-  load_chunk(ca, col(2));
-  store_chunk(cb, 1);
-  load_chunk(cb, col(3));  // chunk 3, stored at the end of walk chunk 0
-  store_chunk(ca, 2);
-  __syncthreads();
+  for (int k = (int)threadIdx.x; k < 99; k += 2 * ZLINES) (&tg_s[0][0])[k] = (&c_tg[0][0])[k];
+  Chunk ch[ZPRE];
   ZWalk st;
-  {
+  if (walker) {
+    Chunk c2;
+    load_chunk(ch[0], 0);
+    load_chunk(c2, 1);
+    store_chunk(ch[0], 0);
+    load_chunk(ch[0], 2);
+    store_chunk(c2, 1);
+    store_chunk(ch[0], 2);
+#pragma unroll
+    for (int u = 0; u < ZPRE; ++u) load_chunk(ch[u], 3 + u);  // stored in iteration u
+    // the walk's start from the ring this wave has just written
     const int sl = ZCH;  // slot of ib
     st.y1 = ring[sl - 1][lane];
     st.y2 = ring[sl - 2][lane];
@@ -465,7 +517,7 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a) {
   auto general_step = [&](int i) {
     const int sl = (i - P0) & (ZRING - 1);
     const double xn = ring[(sl + 9) & (ZRING - 1)][lane];
-    const uint32_t zr = code[lane][sl] & 15u;
+    const uint32_t zr = code(lane, sl) & 15u;
     double w = tg_interior(st);
     if (zr != ROW_INTERIOR) {
       if (zr == ROW_BLANK) {
@@ -475,84 +527,115 @@ __global__ __launch_bounds__(ZLINES) void lad_zrfilter_kernel(LadArgs a) {
         const int base = zr < 4 ? -(int)zr : -(int)zr - 2;
         double acc = 0.0;
 #pragma unroll
-        for (int m = 0; m < 11; ++m) acc = acc + c_tg[zr][m] * ring[(sl + base + m) & (ZRING - 1)][lane];
+        for (int m = 0; m < 11; ++m) acc = acc + tg_s[zr][m] * ring[(sl + base + m) & (ZRING - 1)][lane];
         w = acc;
       }
     }
     ring[sl][lane] = w;
     zwalk_advance(st, w, xn);
   };
-  // the r filter of one column: lane (c16, l4) takes lines out_lo + l4 * per .. of column c16,
-  // interior rows from a sliding window of 9 lines
+  // the r filter of one column: lane (c16, l4) takes lines j0 = out_lo + l4 * per .. j1 - 1 of
+  // column c16.  Interior rows take the interior weights on a sliding window of nine lines (the
+  // next line read one step ahead).  Where the chunk holds a closure or blanked row (rint_s,
+  // colmask_s), the four lanes of a column then share the column's such rows in turn (the first
+  // to lane l4 = 0, ...) with their closure weights, or zero, so the rows at the grid's r ends
+  // take one pass of the wave; the interior pass leaves those points alone.
   const int nout = out_hi - out_lo;
   const int per = (nout + 3) / 4;
-  auto rfilter_chunk = [&](int c0, int n, double ramp) {
+  auto rfilter_chunk = [&](int c0, int n, double ramp, bool allint) {
     const int ii = c0 + c16;
-    if (c16 >= n || ii < S0 || ii >= S1) return;
+    if (c16 >= n || ii < S0 || ii >= S1) return;  // the same for the four lanes of a column
     const int sl = (ii - P0) & (ZRING - 1);
     const int j0 = out_lo + l4 * per;
     const int j1 = min(out_hi, j0 + per);
-    auto val = [&](int j) {  // z-filtered value of line j (zero outside the grid)
-      const int l = j - jb;
-      return (j >= 0 && j < nr && l >= 0 && l < ZLINES) ? ring[sl][l] : 0.0;
-    };
-    // The sliding window of the interior rows reads its lines clamped to the group, without a
-    // test: an interior row's nine lines lie inside the grid and the group (the groups are built
-    // so), so every value it uses was read unclamped; the others use val().
+    const unsigned long long cm = allint ? 0ull : colmask_s[sl];  // bit j - jb: line j is left to the fix-up
+    // An interior row's nine lines lie inside the grid and the group (the groups are built so),
+    // so the clamped reads give every value an interior row uses unclamped.
     auto win = [&](int j) { return ring[sl][min(max(j - jb, 0), ZLINES - 1)]; };
     double w[9];
 #pragma unroll
     for (int m = 0; m < 8; ++m) w[m + 1] = win(j0 - 4 + m);
-    for (int j = j0; j < j1; ++j) {
+    double nx = win(j0 + 4);
+    double* op = out + ((long long)ii + (long long)pitch * j0);
+    auto pass = [&](auto skipping) {
+      unsigned long long skip = cm >> (j0 - jb);
+#pragma unroll 2
+      for (int j = j0; j < j1; ++j) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) w[m] = w[m + 1];
-      w[8] = win(j + 4);
-      const int l = j - jb;
-      const uint32_t rr = code[l][sl] >> 4;
-      double v = 0.0;
-      if (rr == ROW_INTERIOR) {
+        for (int m = 0; m < 8; ++m) w[m] = w[m + 1];
+        w[8] = nx;
+        nx = win(j + 5);
         // symmetric weights paired, fused (a few ulp from the reference's left-to-right sum)
-        v = fma(TG0, w[0] + w[8], fma(TG1, w[1] + w[7], fma(TG2, w[2] + w[6], fma(TG3, w[3] + w[5], TG4 * w[4]))));
-      } else if (rr != ROW_BLANK) {
-        const int base = rr < 4 ? -(int)rr : -(int)rr - 2;
-#pragma unroll
-        for (int m = 0; m < 11; ++m) v = v + c_tg[rr][m] * val(j + base + m);
-      }
-      if (rr != ROW_BLANK) {
+        double v = fma(TG0, w[0] + w[8], fma(TG1, w[1] + w[7], fma(TG2, w[2] + w[6], fma(TG3, w[3] + w[5], TG4 * w[4]))));
         v = C * v;
         v = v * ramp;
+        if (!decltype(skipping)::value || (skip & 1ull) == 0ull) *op = v;
+        skip >>= 1;
+        op += pitch;
       }
-      out[(long long)ii + (long long)pitch * j] = v;
+    };
+    if (allint) pass(std::false_type{});
+    else pass(std::true_type{});
+    if (!allint) {
+      auto val = [&](int j) {  // z-filtered value of line j (zero outside the grid)
+        const int l = j - jb;
+        return (j >= 0 && j < nr && l >= 0 && l < ZLINES) ? ring[sl][l] : 0.0;
+      };
+      // the column's such rows among the group's outputs, every fourth from the l4-th
+      const int lo = out_lo - jb, hi = out_hi - jb;  // 0 <= lo < hi <= 64
+      const unsigned long long outm = (hi >= 64 ? ~0ull : ((1ull << hi) - 1ull)) & ~((1ull << lo) - 1ull);
+      unsigned long long m = cm & outm;
+      for (int q = 0; q < l4; ++q) m &= m - 1ull;
+      while (m != 0ull) {
+        const int j = jb + __builtin_ctzll(m);
+        const uint32_t rr = code(j - jb, sl) >> 4;
+        double v = 0.0;
+        if (rr != ROW_BLANK) {
+          const int base = rr < 4 ? -(int)rr : -(int)rr - 2;
+#pragma unroll
+          for (int q = 0; q < 11; ++q) v = v + tg_s[rr][q] * val(j + base + q);
+          v = C * v;
+          v = v * ramp;
+        }
+        out[(long long)ii + (long long)pitch * j] = v;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) m &= m - 1ull;  // this lane's next row of the column
+      }
     }
   };
-  auto walk_chunk = [&](int k, Chunk& next, const Chunk& ready) {
-    const int c0 = ib + k * ZCH;  // first point of this chunk
-    // the ramp before the chunk loads: waiting for it then does not wait for them
-    const double ramp = a.ramp[min(c0 + c16, nz - 1)];
-    load_chunk(next, col(k + 4));
-    const int n = min(ZCH, S1 - c0);
-    const bool fast = n == ZCH && ((interior >> ((k + 1) & 63)) & 1ull);
-    if (fast) {
-      switch ((k + 1) & 3) {
-        case 0: zwalk_interior<0>(st, ring, lane); break;
-        case 1: zwalk_interior<1>(st, ring, lane); break;
-        case 2: zwalk_interior<2>(st, ring, lane); break;
-        default: zwalk_interior<3>(st, ring, lane); break;
-      }
-    } else {
-      for (int t = 0; t < n; ++t) general_step(c0 + t);
-    }
-    __syncthreads();
-    rfilter_chunk(c0, n, ramp);
-    store_chunk(ready, k + 3);  // overwrites the columns 64 behind, already filtered in r
-    __syncthreads();
-  };
+  __syncthreads();
   const int nchunks = (S1 - ib + ZCH - 1) / ZCH;
-  // chunks in pairs, the second of the last pair possibly empty (n <= 0: no step, no output),
-  // so the loop body has no branch around a walk
-  for (int k = 0; k < nchunks; k += 2) {
-    walk_chunk(k, ca, cb);
-    walk_chunk(k + 1, cb, ca);
+  // iteration k: the walker walks chunk k (k < nchunks) and moves a chunk in, the writer
+  // filters chunk k - 1 in r (k >= 1); both take every iteration's barrier.  The iterations run
+  // in groups of ZPRE, so that the register set each moves is fixed at compile time; those past
+  // k = nchunks only move chunks past the last one read.
+  for (int k0 = 0; k0 <= nchunks; k0 += ZPRE)
+#pragma unroll
+  for (int u = 0; u < ZPRE; ++u) {
+    const int k = k0 + u;
+    if (walker) {
+      if (k < nchunks) {
+        const int c0 = ib + k * ZCH;  // first point of this chunk
+        const int n = min(ZCH, S1 - c0);
+        const bool fast = n == ZCH && ((interior >> ((k + 1) & 63)) & 1ull);
+        if (fast) {
+          switch ((k + 1) & 3) {
+            case 0: zwalk_interior<0>(st, ring, lane); break;
+            case 1: zwalk_interior<1>(st, ring, lane); break;
+            case 2: zwalk_interior<2>(st, ring, lane); break;
+            default: zwalk_interior<3>(st, ring, lane); break;
+          }
+        } else {
+          for (int t = 0; t < n; ++t) general_step(c0 + t);
+        }
+      }
+      store_chunk(ch[u], k + 3);  // over ring chunk k - 1, filtered in r in the iteration before
+      load_chunk(ch[u], k + 3 + ZPRE);
+    } else if (k >= 1) {
+      const int c0 = ib + (k - 1) * ZCH;  // first point of walk chunk k - 1
+      rfilter_chunk(c0, min(ZCH, S1 - c0), ramp_s[(k - 1) & 3][c16], rint_s[k & 3] != 0u);
+    }
+    __syncthreads();
   }
 }
 
@@ -655,7 +738,7 @@ hipError_t launch_lad_publish(const uint32_t* any, uint32_t gen, uint32_t* host,
 int lad_zrfilter_slots(int device) {
   int cus = 0, per_cu = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
-      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lad_zrfilter_kernel, ZLINES, 0) != hipSuccess)
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lad_zrfilter_kernel, 2 * ZLINES, 0) != hipSuccess)
     return 0;
   return cus * per_cu;
 }
@@ -744,7 +827,7 @@ hipError_t launch_lad(const LadArgs& a, hipStream_t s) {
   }
   const int nseg = (g.nz + a.zseg - 1) / a.zseg;
   if (nseg > a.seg_first)
-    hipLaunchKernelGGL(lad_zrfilter_kernel, dim3(nseg - a.seg_first, a.nrgroups, 3), dim3(ZLINES), 0, s, a);
+    hipLaunchKernelGGL(lad_zrfilter_kernel, dim3(nseg - a.seg_first, a.nrgroups, 3), dim3(2 * ZLINES), 0, s, a);
   return hipGetLastError();
 }
 
