@@ -552,15 +552,24 @@ __global__ __launch_bounds__(2 * ZLINES, 2) void lad_zrfilter_kernel(LadArgs a) 
     // An interior row's nine lines lie inside the grid and the group (the groups are built so),
     // so the clamped reads give every value an interior row uses unclamped.
     auto win = [&](int j) { return ring[sl][min(max(j - jb, 0), ZLINES - 1)]; };
-    double w[9];
+    // this lane's rows left to the fix-up (bit t: line j0 + t; per <= 16): those at the two ends
+    // of its range are cut off the interior pass, which then runs without a test per row; rows
+    // between interior ones (walls inside the group) take the pass that tests every row
+    const int span = max(0, j1 - j0);
+    const uint32_t sk = (uint32_t)(cm >> min(j0 - jb, 63)) & ((1u << span) - 1u);
+    const int lead = min(span, __builtin_ctz(~sk));
+    const int trail = span > 0 ? min(span - lead, (int)__builtin_clz(~(sk << (32 - span)))) : 0;
+    const int mid = span - lead - trail;
+    const bool clean = ((sk >> lead) & ((1u << mid) - 1u)) == 0u;
+    auto pass = [&](int ja, int jz, auto skipping) {
+      double w[9];
 #pragma unroll
-    for (int m = 0; m < 8; ++m) w[m + 1] = win(j0 - 4 + m);
-    double nx = win(j0 + 4);
-    double* op = out + ((long long)ii + (long long)pitch * j0);
-    auto pass = [&](auto skipping) {
-      unsigned long long skip = cm >> (j0 - jb);
+      for (int m = 0; m < 8; ++m) w[m + 1] = win(ja - 4 + m);
+      double nx = win(ja + 4);
+      double* op = out + ((long long)ii + (long long)pitch * ja);
+      uint32_t skip = sk;
 #pragma unroll 2
-      for (int j = j0; j < j1; ++j) {
+      for (int j = ja; j < jz; ++j) {
 #pragma unroll
         for (int m = 0; m < 8; ++m) w[m] = w[m + 1];
         w[8] = nx;
@@ -569,13 +578,13 @@ __global__ __launch_bounds__(2 * ZLINES, 2) void lad_zrfilter_kernel(LadArgs a) 
         double v = fma(TG0, w[0] + w[8], fma(TG1, w[1] + w[7], fma(TG2, w[2] + w[6], fma(TG3, w[3] + w[5], TG4 * w[4]))));
         v = C * v;
         v = v * ramp;
-        if (!decltype(skipping)::value || (skip & 1ull) == 0ull) *op = v;
+        if (!decltype(skipping)::value || (skip & 1u) == 0u) *op = v;
         skip >>= 1;
         op += pitch;
       }
     };
-    if (allint) pass(std::false_type{});
-    else pass(std::true_type{});
+    if (clean) pass(j0 + lead, j1 - trail, std::false_type{});
+    else pass(j0, j1, std::true_type{});
     if (!allint) {
       auto val = [&](int j) {  // z-filtered value of line j (zero outside the grid)
         const int l = j - jb;
