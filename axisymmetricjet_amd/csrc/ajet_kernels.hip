@@ -116,6 +116,215 @@ __device__ __forceinline__ double d2i(const double* c, int s) {
   return d2_int(c[-2 * s], c[-s], c[0], c[s], c[2 * s]);
 }
 
+// Pointwise inputs of an interior point, loaded before the stencil work so that their latency
+// hides behind it: the stage update's inputs (stages 2, 3: q^n, and S; stage 4: A and S), the
+// row's eta metrics, r and 1/r, and the target where a sponge or the feedback forcing needs it.
+struct PointIn {
+  double pin0[5], pin1[5];
+  double qt[5];
+  double mr, mr2, r, ir;
+  bool top, inflow;
+};
+
+template <int STAGE, bool FORCE>
+__device__ __forceinline__ void int_inputs(const StageArgs& a, uint32_t boff, int j, double zc, PointIn& in) {
+  const int fsb = (int)(8 * a.g.fstride);
+  in.mr = a.detadr[j];
+  in.mr2 = a.ddetadetadr[j];
+  in.r = a.rc[j];
+  in.ir = a.rinv[j];
+  if (STAGE >= 2) {
+    const Rsrc r0 = rsrc(STAGE == 4 ? a.acc_a : a.qn);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) in.pin0[k] = bld(r0, boff, k * fsb);
+  }
+  if (STAGE >= 3) {
+    const Rsrc r1 = rsrc(a.acc_s);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) in.pin1[k] = bld(r1, boff, k * fsb);
+  }
+  in.top = in.r >= a.pc.sponge_start_top;
+  in.inflow = j >= 1 && zc <= a.pc.sponge_end_left;
+#pragma unroll
+  for (int f = 0; f < 5; ++f) in.qt[f] = 0.0;
+  if (in.top || in.inflow || (FORCE && (a.forcing & AJET_FORCING_FEEDBACK))) {
+    const Rsrc rt = rsrc(a.tgt);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) in.qt[f] = bld(rt, boff, f * fsb);
+  }
+}
+
+// Window accessors of int_point: fields at the point and its SBP interior derivatives (the
+// factored forms of ajet_fastmath.h, before the metric factors), and the staged dVr/dz, dVz/dz.
+// TileWin: the window rows at fixed positions (row stride WZ); c points at field 0 of the point,
+// d at dVr/dz of the point.
+template <int FS, int DS>
+struct TileWin {
+  const double* c;
+  const double* d;
+  __device__ __forceinline__ double at(int f) const { return c[f * FS]; }
+  __device__ __forceinline__ double z1(int f) const { return d1i(c + f * FS, 1); }
+  __device__ __forceinline__ double r1(int f) const { return d1i(c + f * FS, WZ); }
+  __device__ __forceinline__ double z2(int f) const { return d2i(c + f * FS, 1); }
+  __device__ __forceinline__ double r2(int f) const { return d2i(c + f * FS, WZ); }
+  __device__ __forceinline__ double dz(int k) const { return d[k * DS]; }
+  __device__ __forceinline__ double dzr(int k) const { return d1i(d + k * DS, TZ); }
+};
+
+// lambda = beta - (2/3) mu with beta = (3/5) T**0.666 (+ beta*) (ModAssign.f90:21-23,
+// ModShockCapturing.f90:249-250) in the interior tiles' fused form; mu_phys = T**0.666.
+template <bool LAD>
+__device__ __forceinline__ double int_lambda(double mu, double mu_phys, double beta_star) {
+  double beta = 0.6 * mu_phys;
+  if (LAD) beta = beta + beta_star;
+  return fma(-2.0 / 3.0, mu, beta);
+}
+
+// The stage update's pointwise inputs held in registers (PointIn).
+struct PinRegs {
+  const PointIn& in;
+  __device__ __forceinline__ double p0(int k) const { return in.pin0[k]; }
+  __device__ __forceinline__ double p1(int k) const { return in.pin1[k]; }
+};
+// One interior point of a stage with the fused forms (ajet_fastmath.h, inviscid_fast,
+// viscous_fast): derivatives from the window W, forcing, sponges, the stage update (its inputs
+// from P) and its stores at byte offset boff; stage 4 folds the point's CFL denominator into
+// dmax.  mz, mz2, zc: the column's xi metrics and z; in: the row's metrics and the target.
+template <int STAGE, int MODE, bool FORCE, class Win, class Pins>
+__device__ __forceinline__ void int_point(const StageArgs& a, uint32_t boff, const Win& W, double mz, double mz2,
+                                          double zc, double dt, double sTst, double tst, const PointIn& in,
+                                          const Pins& P, double& dmax) {
+  constexpr bool LAD = MODE == MODE_LAD;
+  constexpr bool VISC = MODE != MODE_INVISCID;
+  constexpr bool SC = MODE == MODE_SC || MODE == MODE_LAD;
+  constexpr int FK = 7, FM = 8;  // LAD: kappa and the physical mu
+  const int fsb = (int)(8 * a.g.fstride);
+  const double mr = in.mr, mr2 = in.mr2, r = in.r, ir = in.ir;
+
+  double q[5];
+#pragma unroll
+  for (int f = 0; f < 5; ++f) q[f] = W.at(f);
+
+  PointDerivs d;
+  double rhs[5];
+  const double mu = W.at(6);
+  const double irho = rcp(q[0]);
+  d.drhodz = W.z1(0) * mz;
+  d.dVrdz = W.dz(0);
+  d.dVtdz = W.z1(2) * mz;
+  d.dVzdz = W.dz(1);
+  d.dTdz = W.z1(4) * mz;
+  d.dpdz = W.z1(5) * mz;
+  d.drhodr = W.r1(0) * mr;
+  d.dVrdr = W.r1(1) * mr;
+  d.dVtdr = W.r1(2) * mr;
+  d.dVzdr = W.r1(3) * mr;
+  d.dTdr = W.r1(4) * mr;
+  d.dpdr = W.r1(5) * mr;
+  inviscid_fast(q, ir, irho, d, a.pc, rhs);
+  if (VISC) {
+    d.dmudz = W.z1(6) * mz;
+    d.dmudr = W.r1(6) * mr;
+    double kappa = mu;
+    if constexpr (LAD) {
+      kappa = W.at(FK);
+      d.dkappadz = W.z1(FK) * mz;
+      d.dkappadr = W.r1(FK) * mr;
+    } else {
+      d.dkappadz = d.dmudz;
+      d.dkappadr = d.dmudr;
+    }
+    const double z2 = mz * mz, r2 = mr * mr;
+    d.d2Vrdz2 = fma(W.z2(1), z2, d.dVrdz * mz2);
+    d.d2Vtdz2 = fma(W.z2(2), z2, d.dVtdz * mz2);
+    d.d2Vzdz2 = fma(W.z2(3), z2, d.dVzdz * mz2);
+    d.d2Tdz2 = fma(W.z2(4), z2, d.dTdz * mz2);
+    d.d2Vrdr2 = fma(W.r2(1), r2, d.dVrdr * mr2);
+    d.d2Vtdr2 = fma(W.r2(2), r2, d.dVtdr * mr2);
+    d.d2Vzdr2 = fma(W.r2(3), r2, d.dVzdr * mr2);
+    d.d2Tdr2 = fma(W.r2(4), r2, d.dTdr * mr2);
+    d.d2Vrdzdr = W.dzr(0) * mr;
+    d.d2Vzdzdr = W.dzr(1) * mr;
+    double lambda = 0.0;
+    if (SC) {
+      lambda = int_lambda<LAD>(mu, LAD ? W.at(FM) : mu, LAD ? bld(rsrc(a.lad), boff, fsb) : 0.0);
+    }
+    viscous_fast(q, ir, irho, mu, kappa, lambda, SC, d, a.pc, rhs);
+  }
+
+  // forcing (ModRHS.f90:223-225), at every point, before the boundary conditions
+  if (FORCE) {
+    const double dzf = zc - a.fc.z0, drf = r - a.fc.r0;
+    const double G = exp(-1.0 * ((dzf * dzf) / a.fc.lz2) - (drf * drf) / a.fc.lr2);
+    if (a.forcing & AJET_FORCING_FEEDBACK) {
+      // rhs(e) += alpha*forcing_mat(e,k)*GaussianFactor*(pv(k) - pv_target(k)), k in order
+#pragma unroll
+      for (int e = 0; e < 5; ++e)
+#pragma unroll
+        for (int k = 0; k < 5; ++k) rhs[e] = rhs[e] + (a.fc.aF[e][k] * G) * (q[k] - in.qt[k]);
+    }
+    if (a.forcing & AJET_FORCING_EXCITATION) rhs[3] = rhs[3] + (a.fc.au * sin(a.fc.w * (a.fc.t0 + tst))) * G;
+  }
+
+  // the sponges (ModBC.f90:548-617); an interior point carries no boundary patch
+  if (in.top) {
+    const double zeta = (r - a.pc.sponge_start_top) / (a.pc.r_top - a.pc.sponge_start_top);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) rhs[k] = rhs[k] - a.pc.sponge_A_top * (zeta * zeta) * (q[k] - in.qt[k]);
+  }
+  if (in.inflow) {
+    const double zeta = (a.pc.sponge_end_left - zc) / a.pc.sponge_end_left;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) rhs[k] = rhs[k] - a.pc.sponge_A_left * (zeta * zeta) * (q[k] - in.qt[k]);
+  }
+
+  if (STAGE == 0) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) bst(rsrc(a.rhs_out), boff, k * fsb, rhs[k]);
+    return;
+  }
+
+  // ---- stage update (ModRungeKutta.f90:32-146), k = dt rhs folded into one fused multiply-add
+  // per value
+  const double c16 = 1.0 / 6.0, c13 = 1.0 / 3.0;
+  double out[5], acc[5];
+  const double hdt = 0.5 * dt, dt6 = c16 * dt;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    if (STAGE == 1) {
+      out[k] = fma(hdt, rhs[k], q[k]);
+      acc[k] = fma(dt6, rhs[k], q[k]);
+    } else if (STAGE == 2) {
+      out[k] = fma(hdt, rhs[k], P.p0(k));
+      acc[k] = dt * rhs[k];
+    } else if (STAGE == 3) {
+      out[k] = fma(dt, rhs[k], P.p0(k));
+      acc[k] = fma(dt, rhs[k], P.p1(k));
+    } else {
+      out[k] = fma(dt6, rhs[k], fma(c13, P.p1(k), P.p0(k)));
+    }
+  }
+  if (STAGE == 1) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) bst(rsrc(a.acc_a), boff, k * fsb, acc[k]);
+  } else if (STAGE == 2 || STAGE == 3) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) bst(rsrc(a.acc_s), boff, k * fsb, acc[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < 5; ++k) bst(rsrc(a.qout), boff, k * fsb, out[k]);
+  if (STAGE == 4) {
+    // ComputeTimeStep's denominator of the new state (lambda / Re never wins: lambda < 0)
+    const double T = out[4];
+    const double mun = pow0666(T);
+    const double inv = fabs(fma(out[3], mz, out[1] * mr)) + sTst * sqrt(fma(mz, mz, mr * mr));
+    const double sm = fabs(mz) + fabs(mr);
+    const double den = fma(2.0 * (mun * a.pc.cfl_visc), sm * sm, inv);
+    if (!(den <= 1.0e300)) a.sc->nonfinite = 1;
+    dmax = fmax(dmax, den);
+  }
+}
+
 template <int STAGE, int MODE, bool INT, bool FORCE>
 __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool writer, int dt_from_dmax, int parity,
                                            double (&sF)[stage_nf<MODE>()][WR][WZ], double (&sD)[2][WR][TZ],
@@ -266,10 +475,17 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     const int i = i0 + tz, j = j0 + ty;
     if (i < oi_lo || i >= oi_hi || j < oj_lo || j >= oj_hi) continue;
     const int wz = tz + HALO, wr = ty + HALO;
+    if constexpr (INT) {
+      const uint32_t boff = 8u * (uint32_t)(i + pitch * j);
+      PointIn in;
+      int_inputs<STAGE, FORCE>(a, boff, j, zcol, in);
+      const TileWin<WR * WZ, WR * TZ> W{&sF[0][wr][wz], &sD[0][wr][tz]};
+      int_point<STAGE, MODE, FORCE>(a, boff, W, mcol, mcol2, zcol, dt, sTst, tst, in, PinRegs{in}, dmax);
+      continue;
+    }
     const long long gi = (long long)i + (long long)pitch * j;
-    const uint32_t boff = 8u * (uint32_t)(i + pitch * j);
-    const uint32_t meta = INT ? ROW_INTERIOR | (ROW_INTERIOR << 4) : a.meta[gi];
-    const uint32_t zr = INT ? ROW_INTERIOR : meta & 15u, rr = INT ? ROW_INTERIOR : (meta >> 4) & 15u;
+    const uint32_t meta = a.meta[gi];
+    const uint32_t zr = meta & 15u, rr = (meta >> 4) & 15u;
     const double mz = mcol, mz2 = mcol2;  // the column's metrics, loaded with the window
     const double mr = a.detadr[j], mr2 = a.ddetadetadr[j];
     const double r = a.rc[j];
@@ -280,21 +496,21 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     if (STAGE >= 2) {
       const double* __restrict__ src0 = STAGE == 4 ? a.acc_a : a.qn;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) pin0[k] = INT ? bld(rsrc(src0), boff, k * fsb) : src0[gi + k * fs];
+      for (int k = 0; k < 5; ++k) pin0[k] = src0[gi + k * fs];
     }
     if (STAGE >= 3) {
       const double* __restrict__ src1 = a.acc_s;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) pin1[k] = INT ? bld(rsrc(src1), boff, k * fsb) : src1[gi + k * fs];
+      for (int k = 0; k < 5; ++k) pin1[k] = src1[gi + k * fs];
     }
     // the target where a boundary patch or a sponge needs it
-    const uint32_t sid = INT ? 0u : (meta >> 8) & 255u;
+    const uint32_t sid = (meta >> 8) & 255u;
     const bool top = r >= a.pc.sponge_start_top;
     const bool inflow = j >= 1 && zc <= a.pc.sponge_end_left;
     double qt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (sid != 0 || top || inflow || (FORCE && (a.forcing & AJET_FORCING_FEEDBACK))) {
 #pragma unroll
-      for (int f = 0; f < 5; ++f) qt[f] = INT ? bld(rsrc(a.tgt), boff, f * fsb) : a.tgt[gi + f * fs];
+      for (int f = 0; f < 5; ++f) qt[f] = a.tgt[gi + f * fs];
     }
 
     double q[5];
@@ -304,116 +520,66 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     PointDerivs d;
     double rhs[5];
     double mu = sF[6][wr][wz];
-    if constexpr (INT) {
-      const double ir = a.rinv[j];
-      const double irho = rcp(q[0]);
-      d.drhodz = d1i(&sF[0][wr][wz], 1) * mz;
-      d.dVrdz = sD[0][wr][tz];
-      d.dVtdz = d1i(&sF[2][wr][wz], 1) * mz;
-      d.dVzdz = sD[1][wr][tz];
-      d.dTdz = d1i(&sF[4][wr][wz], 1) * mz;
-      d.dpdz = d1i(&sF[5][wr][wz], 1) * mz;
-      d.drhodr = d1i(&sF[0][wr][wz], WZ) * mr;
-      d.dVrdr = d1i(&sF[1][wr][wz], WZ) * mr;
-      d.dVtdr = d1i(&sF[2][wr][wz], WZ) * mr;
-      d.dVzdr = d1i(&sF[3][wr][wz], WZ) * mr;
-      d.dTdr = d1i(&sF[4][wr][wz], WZ) * mr;
-      d.dpdr = d1i(&sF[5][wr][wz], WZ) * mr;
-      inviscid_fast(q, ir, irho, d, a.pc, rhs);
-      if (VISC) {
-        d.dmudz = d1i(&sF[6][wr][wz], 1) * mz;
-        d.dmudr = d1i(&sF[6][wr][wz], WZ) * mr;
-        double kappa = mu;
-        if constexpr (LAD) {
-          kappa = sF[FK][wr][wz];
-          d.dkappadz = d1i(&sF[FK][wr][wz], 1) * mz;
-          d.dkappadr = d1i(&sF[FK][wr][wz], WZ) * mr;
-        } else {
-          d.dkappadz = d.dmudz;
-          d.dkappadr = d.dmudr;
-        }
-        const double z2 = mz * mz, r2 = mr * mr;
-        d.d2Vrdz2 = fma(d2i(&sF[1][wr][wz], 1), z2, d.dVrdz * mz2);
-        d.d2Vtdz2 = fma(d2i(&sF[2][wr][wz], 1), z2, d.dVtdz * mz2);
-        d.d2Vzdz2 = fma(d2i(&sF[3][wr][wz], 1), z2, d.dVzdz * mz2);
-        d.d2Tdz2 = fma(d2i(&sF[4][wr][wz], 1), z2, d.dTdz * mz2);
-        d.d2Vrdr2 = fma(d2i(&sF[1][wr][wz], WZ), r2, d.dVrdr * mr2);
-        d.d2Vtdr2 = fma(d2i(&sF[2][wr][wz], WZ), r2, d.dVtdr * mr2);
-        d.d2Vzdr2 = fma(d2i(&sF[3][wr][wz], WZ), r2, d.dVzdr * mr2);
-        d.d2Tdr2 = fma(d2i(&sF[4][wr][wz], WZ), r2, d.dTdr * mr2);
-        d.d2Vrdzdr = d1i(&sD[0][wr][tz], TZ) * mr;
-        d.d2Vzdzdr = d1i(&sD[1][wr][tz], TZ) * mr;
+    d.drhodz = d1_sum(zr, &sF[0][wr][wz], 1) * mz;
+    d.dVrdz = sD[0][wr][tz];
+    d.dVtdz = d1_sum(zr, &sF[2][wr][wz], 1) * mz;
+    d.dVzdz = sD[1][wr][tz];
+    d.dTdz = d1_sum(zr, &sF[4][wr][wz], 1) * mz;
+    d.dpdz = d1_sum(zr, &sF[5][wr][wz], 1) * mz;
+    AJET_FENCE();
+    d.drhodr = d1_sum(rr, &sF[0][wr][wz], WZ) * mr;
+    d.dVrdr = d1_sum(rr, &sF[1][wr][wz], WZ) * mr;
+    d.dVtdr = d1_sum(rr, &sF[2][wr][wz], WZ) * mr;
+    d.dVzdr = d1_sum(rr, &sF[3][wr][wz], WZ) * mr;
+    d.dTdr = d1_sum(rr, &sF[4][wr][wz], WZ) * mr;
+    d.dpdr = d1_sum(rr, &sF[5][wr][wz], WZ) * mr;
+    AJET_FENCE();
+    if (j == 0) {
+      // the axis row is skipped by the interior loops (ModRHS.f90:59-67): rhs = -0
+#pragma unroll
+      for (int k = 0; k < 5; ++k) rhs[k] = -1.0 * 0.0;
+    } else {
+      inviscid(q, r, d, a.pc.gam, rhs);
+    }
+    AJET_FENCE();
+    if (VISC) {
+      d.dmudz = d1_sum(zr, &sF[6][wr][wz], 1) * mz;
+      d.dmudr = d1_sum(rr, &sF[6][wr][wz], WZ) * mr;
+      double kappa = mu;
+      if constexpr (LAD) {
+        kappa = sF[FK][wr][wz];
+        d.dkappadz = d1_sum(zr, &sF[FK][wr][wz], 1) * mz;
+        d.dkappadr = d1_sum(rr, &sF[FK][wr][wz], WZ) * mr;
+      } else {
+        d.dkappadz = d.dmudz;
+        d.dkappadr = d.dmudr;
+      }
+      const double z2 = mz * mz, r2 = mr * mr;
+      AJET_FENCE();
+      d.d2Vrdz2 = d2_sum(zr, &sF[1][wr][wz], 1) * z2 + d.dVrdz * mz2;
+      d.d2Vtdz2 = d2_sum(zr, &sF[2][wr][wz], 1) * z2 + d.dVtdz * mz2;
+      d.d2Vzdz2 = d2_sum(zr, &sF[3][wr][wz], 1) * z2 + d.dVzdz * mz2;
+      d.d2Tdz2 = d2_sum(zr, &sF[4][wr][wz], 1) * z2 + d.dTdz * mz2;
+      AJET_FENCE();
+      d.d2Vrdr2 = d2_sum(rr, &sF[1][wr][wz], WZ) * r2 + d.dVrdr * mr2;
+      d.d2Vtdr2 = d2_sum(rr, &sF[2][wr][wz], WZ) * r2 + d.dVtdr * mr2;
+      d.d2Vzdr2 = d2_sum(rr, &sF[3][wr][wz], WZ) * r2 + d.dVzdr * mr2;
+      d.d2Tdr2 = d2_sum(rr, &sF[4][wr][wz], WZ) * r2 + d.dTdr * mr2;
+      AJET_FENCE();
+      d.d2Vrdzdr = d1_sum(rr, &sD[0][wr][tz], TZ) * mr;
+      d.d2Vzdzdr = d1_sum(rr, &sD[1][wr][tz], TZ) * mr;
+      AJET_FENCE();
+      if (j != 0) {
         double lambda = 0.0;
         if (SC) {
+          // beta = (3/5) T**0.666 (+ beta*), lambda = beta - (2/3) mu (ModAssign.f90:21-23,
+          // ModShockCapturing.f90:249-250); without the artificial terms mu is T**0.666 itself
           const double mu_phys = LAD ? sF[FM][wr][wz] : mu;
-          double beta = 0.6 * mu_phys;
-          if (LAD) beta = beta + bld(rsrc(a.lad), boff, fsb);
-          lambda = fma(-2.0 / 3.0, mu, beta);
+          double beta = (3.0 / 5.0) * mu_phys;
+          if (LAD) beta = beta + a.lad[gi + fs];
+          lambda = beta - (2.0 / 3.0) * mu;
         }
-        viscous_fast(q, ir, irho, mu, kappa, lambda, SC, d, a.pc, rhs);
-      }
-    } else {
-      d.drhodz = d1_sum(zr, &sF[0][wr][wz], 1) * mz;
-      d.dVrdz = sD[0][wr][tz];
-      d.dVtdz = d1_sum(zr, &sF[2][wr][wz], 1) * mz;
-      d.dVzdz = sD[1][wr][tz];
-      d.dTdz = d1_sum(zr, &sF[4][wr][wz], 1) * mz;
-      d.dpdz = d1_sum(zr, &sF[5][wr][wz], 1) * mz;
-      AJET_FENCE();
-      d.drhodr = d1_sum(rr, &sF[0][wr][wz], WZ) * mr;
-      d.dVrdr = d1_sum(rr, &sF[1][wr][wz], WZ) * mr;
-      d.dVtdr = d1_sum(rr, &sF[2][wr][wz], WZ) * mr;
-      d.dVzdr = d1_sum(rr, &sF[3][wr][wz], WZ) * mr;
-      d.dTdr = d1_sum(rr, &sF[4][wr][wz], WZ) * mr;
-      d.dpdr = d1_sum(rr, &sF[5][wr][wz], WZ) * mr;
-      AJET_FENCE();
-      if (j == 0) {
-        // the axis row is skipped by the interior loops (ModRHS.f90:59-67): rhs = -0
-#pragma unroll
-        for (int k = 0; k < 5; ++k) rhs[k] = -1.0 * 0.0;
-      } else {
-        inviscid(q, r, d, a.pc.gam, rhs);
-      }
-      AJET_FENCE();
-      if (VISC) {
-        d.dmudz = d1_sum(zr, &sF[6][wr][wz], 1) * mz;
-        d.dmudr = d1_sum(rr, &sF[6][wr][wz], WZ) * mr;
-        double kappa = mu;
-        if constexpr (LAD) {
-          kappa = sF[FK][wr][wz];
-          d.dkappadz = d1_sum(zr, &sF[FK][wr][wz], 1) * mz;
-          d.dkappadr = d1_sum(rr, &sF[FK][wr][wz], WZ) * mr;
-        } else {
-          d.dkappadz = d.dmudz;
-          d.dkappadr = d.dmudr;
-        }
-        const double z2 = mz * mz, r2 = mr * mr;
-        AJET_FENCE();
-        d.d2Vrdz2 = d2_sum(zr, &sF[1][wr][wz], 1) * z2 + d.dVrdz * mz2;
-        d.d2Vtdz2 = d2_sum(zr, &sF[2][wr][wz], 1) * z2 + d.dVtdz * mz2;
-        d.d2Vzdz2 = d2_sum(zr, &sF[3][wr][wz], 1) * z2 + d.dVzdz * mz2;
-        d.d2Tdz2 = d2_sum(zr, &sF[4][wr][wz], 1) * z2 + d.dTdz * mz2;
-        AJET_FENCE();
-        d.d2Vrdr2 = d2_sum(rr, &sF[1][wr][wz], WZ) * r2 + d.dVrdr * mr2;
-        d.d2Vtdr2 = d2_sum(rr, &sF[2][wr][wz], WZ) * r2 + d.dVtdr * mr2;
-        d.d2Vzdr2 = d2_sum(rr, &sF[3][wr][wz], WZ) * r2 + d.dVzdr * mr2;
-        d.d2Tdr2 = d2_sum(rr, &sF[4][wr][wz], WZ) * r2 + d.dTdr * mr2;
-        AJET_FENCE();
-        d.d2Vrdzdr = d1_sum(rr, &sD[0][wr][tz], TZ) * mr;
-        d.d2Vzdzdr = d1_sum(rr, &sD[1][wr][tz], TZ) * mr;
-        AJET_FENCE();
-        if (j != 0) {
-          double lambda = 0.0;
-          if (SC) {
-            // beta = (3/5) T**0.666 (+ beta*), lambda = beta - (2/3) mu (ModAssign.f90:21-23,
-            // ModShockCapturing.f90:249-250); without the artificial terms mu is T**0.666 itself
-            const double mu_phys = LAD ? sF[FM][wr][wz] : mu;
-            double beta = (3.0 / 5.0) * mu_phys;
-            if (LAD) beta = beta + a.lad[gi + fs];
-            lambda = beta - (2.0 / 3.0) * mu;
-          }
-          viscous(q, r, mu, kappa, lambda, SC, d, a.pc, rhs);
-        }
+        viscous(q, r, mu, kappa, lambda, SC, d, a.pc, rhs);
       }
     }
 
@@ -462,45 +628,6 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     // ---- stage update (ModRungeKutta.f90:32-146), from the inputs loaded at the top
     double out[5], acc[5];
     const double c16 = 1.0 / 6.0, c13 = 1.0 / 3.0;
-    if constexpr (INT) {
-      // k = dt rhs folded into one fused multiply-add per value
-      const double hdt = 0.5 * dt, dt6 = c16 * dt;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        if (STAGE == 1) {
-          out[k] = fma(hdt, rhs[k], q[k]);
-          acc[k] = fma(dt6, rhs[k], q[k]);
-        } else if (STAGE == 2) {
-          out[k] = fma(hdt, rhs[k], pin0[k]);
-          acc[k] = dt * rhs[k];
-        } else if (STAGE == 3) {
-          out[k] = fma(dt, rhs[k], pin0[k]);
-          acc[k] = fma(dt, rhs[k], pin1[k]);
-        } else {
-          out[k] = fma(dt6, rhs[k], fma(c13, pin1[k], pin0[k]));
-        }
-      }
-      if (STAGE == 1) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) bst(rsrc(a.acc_a), boff, k * fsb, acc[k]);
-      } else if (STAGE == 2 || STAGE == 3) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k) bst(rsrc(a.acc_s), boff, k * fsb, acc[k]);
-      }
-#pragma unroll
-      for (int k = 0; k < 5; ++k) bst(rsrc(a.qout), boff, k * fsb, out[k]);
-      if (STAGE == 4) {
-        // ComputeTimeStep's denominator of the new state (lambda / Re never wins: lambda < 0)
-        const double T = out[4];
-        const double mun = pow0666(T);
-        const double inv = fabs(fma(out[3], mz, out[1] * mr)) + sTst * sqrt(fma(mz, mz, mr * mr));
-        const double sm = fabs(mz) + fabs(mr);
-        const double den = fma(2.0 * (mun * a.pc.cfl_visc), sm * sm, inv);
-        if (!(den <= 1.0e300)) a.sc->nonfinite = 1;
-        dmax = fmax(dmax, den);
-      }
-      continue;
-    }
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       const double kk = dt * rhs[k];
