@@ -38,6 +38,7 @@ struct ajet_ctx {
   int n_int = 0, n_bnd = 0;
   int* mix_tiles = nullptr;  // both in one list (mixed_tile_list), for the LAD-mode stage kernel
   int n_mix = 0;
+  int4 *int_box = nullptr, *bnd_box = nullptr, *mix_box = nullptr;  // the three lists' tile bounds
   int* lad_tiles = nullptr;  // the strain and source kernels' tiles, split the same way
   int n_lad_tiles = 0;
   int* lad_groups = nullptr;                       // line groups of the fused z/r filter
@@ -154,7 +155,8 @@ int ajet_destroy(ajet_ctx* c) {
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
                   c->int_tiles, c->bnd_tiles, c->mix_tiles, c->lad_tiles, c->lad_groups, c->codes, c->sc, c->dtlog,
                   c->mlpW, c->mlpB, c->mlpShift, c->mlpScale, c->zr_flags, c->src_mask, c->str_mask,
-                  c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out, c->any_dev, c->lad_net, c->rinv};
+                  c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out, c->any_dev, c->lad_net, c->rinv,
+                  c->int_box, c->bnd_box, c->mix_box};
   if (c->aux) (void)hipStreamSynchronize(c->aux);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -206,6 +208,9 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   g.iT = c->s.iT;
   g.jT = c->s.jT;
   g.nseq = c->s.nseq;
+  g.jtop = nr;
+  for (int j = nr - 1; j >= 0; --j)
+    if (c->s.rc[(size_t)j] >= p->sponge_start_top) g.jtop = j;
   c->pc.gam = p->gam;
   c->pc.Re = p->Re;
   c->pc.Pr = p->Pr;
@@ -297,6 +302,21 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
     c->n_mix = (int)mix.size();
     HIPCHK(c, hipMalloc((void**)&c->mix_tiles, sizeof(int) * mix.size()));
     HIPCHK(c, hipMemcpy(c->mix_tiles, mix.data(), sizeof(int) * mix.size(), hipMemcpyHostToDevice));
+    // each list entry's owned columns and rows, so that a block needs one load to find its tile
+    const int ntz = g.ntz;
+    auto boxes = [&](const std::vector<int>& list, int4** dst) -> hipError_t {
+      std::vector<int4> b(list.size() + 1, make_int4(0, 0, 0, 0));
+      for (size_t k = 0; k < list.size(); ++k) {
+        const int t = list[k] >= 0 ? list[k] : ~list[k];
+        b[k] = make_int4(c->s.zcuts[(size_t)(t % ntz)], c->s.zcuts[(size_t)(t % ntz) + 1], c->s.rcuts[(size_t)(t / ntz)],
+                         c->s.rcuts[(size_t)(t / ntz) + 1]);
+      }
+      hipError_t e = hipMalloc((void**)dst, sizeof(int4) * b.size());
+      return e == hipSuccess ? hipMemcpy(*dst, b.data(), sizeof(int4) * b.size(), hipMemcpyHostToDevice) : e;
+    };
+    HIPCHK(c, boxes(it, &c->int_box));
+    HIPCHK(c, boxes(bt, &c->bnd_box));
+    HIPCHK(c, boxes(mix, &c->mix_box));
     classify_lad_tiles(c->s, nz, nr, it, bt);
     mixed_tile_list(it, bt, mix);
     c->n_lad_tiles = (int)mix.size();
@@ -586,6 +606,7 @@ static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
     // one launch for both tile classes: an event between two streams costs ~6 us of idle GPU
     // each way, and the stage kernel holds 2 waves per SIMD here for either class
     a.tiles = c->mix_tiles;
+    a.boxes = c->mix_box;
     a.ntiles = c->n_mix;
     return launch_stage_mixed(stage, mode, a, c->stream);
   }
@@ -595,19 +616,23 @@ static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
     // stage 2 141 -> 128 us at 4400 x 640); stage 4, whose boundary tiles take longer, keeps
     // them beside the interior tiles on the second stream
     a.tiles = c->int_tiles;
+    a.boxes = c->int_box;
     a.ntiles = c->n_int;
     if ((e = launch_stage(stage, mode, true, a, c->stream)) != hipSuccess) return e;
     a.tiles = c->bnd_tiles;
+    a.boxes = c->bnd_box;
     a.ntiles = c->n_bnd;
     return launch_stage(stage, mode, false, a, c->stream);
   }
   if ((e = hipEventRecord(c->ev_fork, c->stream)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->aux, c->ev_fork, 0)) != hipSuccess) return e;
   a.tiles = c->bnd_tiles;
+  a.boxes = c->bnd_box;
   a.ntiles = c->n_bnd;
   if ((e = launch_stage(stage, mode, false, a, c->aux)) != hipSuccess) return e;
   if ((e = hipEventRecord(c->ev_join, c->aux)) != hipSuccess) return e;
   a.tiles = c->int_tiles;
+  a.boxes = c->int_box;
   a.ntiles = c->n_int;
   if ((e = launch_stage(stage, mode, true, a, c->stream)) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(c->stream, c->ev_join, 0)) != hipSuccess) return e;

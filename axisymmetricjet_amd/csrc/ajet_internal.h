@@ -67,6 +67,7 @@ struct Geometry {
   int any_blank;
   int iT, jT;             // 0-based point whose T becomes the stale scalar after a step (-1: none)
   int nseq;
+  int jtop;               // first row whose r reaches the top sponge (nr: none)
 };
 
 // Host-side setup products (pure CPU, shared with tests through the C ABI).

@@ -37,6 +37,7 @@ struct StageArgs {
   const int* zcuts;        // tile boundaries along z (ntz + 1 entries)
   const int* rcuts;        // tile boundaries along r (ntr + 1 entries)
   const int* tiles;        // tiles of this launch (id = tri * ntz + tzi)
+  const int4* boxes;       // their owned columns [x, y) and rows [z, w) (single-run launches)
   int ntiles;
   StepScalars* sc;
   Geometry g;

@@ -4,6 +4,8 @@
 // multiply-adds are the explicit fma() calls of the interior-tile forms.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "ajet_kernels.h"
 #include "ajet_physics.h"
 #include "ajet_stencil.h"
@@ -19,13 +21,6 @@ namespace ajet {
 constexpr int STAGE_WAVES_PER_EU = AJET_STAGE_WAVES;
 
 #define AJET_FENCE() do { } while (0)
-
-// Tile t owns [cuts[t], cuts[t+1]) (at most TZ or TR points); its window starts HALO before.
-__device__ __forceinline__ void tile_range(int t, const int* cuts, int& lo, int& own_lo, int& own_hi) {
-  own_lo = cuts[t];
-  own_hi = cuts[t + 1];
-  lo = own_lo;
-}
 
 __device__ __forceinline__ double warp_max(double v) {
 #pragma unroll
@@ -91,6 +86,27 @@ constexpr int stage_nf() { return MODE == MODE_LAD ? 9 : 7; }
   __shared__ double sF[stage_nf<MODE>()][WR][WZ];             \
   __shared__ double sD[2][WR][TZ];                            \
   __shared__ double red[STAGE_THREADS / 64]
+
+// What a boundary tile keeps in LDS besides its window, loaded with the window so that no
+// per-point step waits on a dependent load: the window's metadata words, the ordered patch lists
+// and the patches.  Interior tiles need none of it.
+struct BndLds {
+  uint32_t meta[WR][WZ];
+  uint32_t seq[MAX_SEQ * SEQ_LEN / 4];  // the bytes of the patch lists, four to a word
+  PatchDev patches[MAX_PATCH];
+};
+struct NoLds {
+  char unused;
+};
+template <bool INT>
+using StageBndLds = std::conditional_t<INT, NoLds, BndLds>;
+static_assert(MAX_SEQ * SEQ_LEN % 4 == 0, "the patch lists fill whole words");
+__device__ __forceinline__ uint32_t bl_meta(const BndLds& b, int wr, int wz) { return b.meta[wr][wz]; }
+__device__ __forceinline__ uint32_t bl_meta(const NoLds&, int, int) { return ROW_INTERIOR | (ROW_INTERIOR << 4); }
+__device__ __forceinline__ int bl_seq(const BndLds& b, int k) { return reinterpret_cast<const uint8_t*>(b.seq)[k]; }
+__device__ __forceinline__ int bl_seq(const NoLds&, int) { return 0; }
+__device__ __forceinline__ PatchDev bl_patch(const BndLds& b, int k) { return b.patches[k]; }
+__device__ __forceinline__ PatchDev bl_patch(const NoLds&, int) { return PatchDev{}; }
 
 // Interior tiles address fields through raw buffer descriptors: one 32-bit byte offset per
 // point (voffset) shared by every field, the field's byte offset f * fstride * 8 in a scalar
@@ -325,10 +341,11 @@ __device__ __forceinline__ void int_point(const StageArgs& a, uint32_t boff, con
   }
 }
 
+// The stage on one tile: owned columns [box.x, box.y), rows [box.z, box.w).
 template <int STAGE, int MODE, bool INT, bool FORCE>
-__device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool writer, int dt_from_dmax, int parity,
+__device__ __forceinline__ void stage_body(const StageArgs& a, int4 box, bool writer, int dt_from_dmax, int parity,
                                            double (&sF)[stage_nf<MODE>()][WR][WZ], double (&sD)[2][WR][TZ],
-                                           double (&red)[STAGE_THREADS / 64]) {
+                                           double (&red)[STAGE_THREADS / 64], StageBndLds<INT>& bl) {
   constexpr bool LAD = MODE == MODE_LAD;
   constexpr bool VISC = MODE != MODE_INVISCID;
   constexpr bool SC = MODE == MODE_SC || MODE == MODE_LAD;
@@ -338,10 +355,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
   const int nz = g.nz, nr = g.nr, pitch = g.pitch;
   const long long fs = g.fstride;
   const int fsb = (int)(8 * fs);  // a field's byte stride (interior tiles' buffer offsets)
-  const int tzi = t % g.ntz, tri = t / g.ntz;
-  int i0, oi_lo, oi_hi, j0, oj_lo, oj_hi;
-  tile_range(tzi, a.zcuts, i0, oi_lo, oi_hi);
-  tile_range(tri, a.rcuts, j0, oj_lo, oj_hi);
+  const int i0 = box.x, oi_lo = box.x, oi_hi = box.y, j0 = box.z, oj_lo = box.z, oj_hi = box.w;
 
   // ---- stage the window: primitives, p = rho T, mu = T**0.666 (AssignValues).  Every load
   // of the thread's window points (with LAD also mu* and kappa*) is issued before any of them
@@ -387,6 +401,18 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
         wlad[u][0] = a.lad[gi];
         wlad[u][1] = a.lad[gi + 2 * fs];
       }
+      if (k < WZ * WR) bl.meta[wr][wz] = inside[u] ? a.meta[gi] : (ROW_BLANK | (ROW_BLANK << 4));
+    }
+  }
+  if constexpr (!INT) {
+    // the patch lists and the patches (every list, every patch slot)
+    constexpr int NSW = MAX_SEQ * SEQ_LEN / 4, NPW = MAX_PATCH * (int)(sizeof(PatchDev) / 4);
+    static_assert(NSW + NPW <= 2 * STAGE_THREADS, "two words per thread");
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int k = threadIdx.x + u * STAGE_THREADS;
+      if (k < NSW) bl.seq[k] = reinterpret_cast<const uint32_t*>(a.seq)[k];
+      else if (k < NSW + NPW) reinterpret_cast<int32_t*>(bl.patches)[k - NSW] = reinterpret_cast<const int32_t*>(a.patches)[k - NSW];
     }
   }
 #pragma unroll
@@ -428,7 +454,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
         dvz = d1i(&sF[3][wr][tz + HALO], 1) * mcol;
       }
     } else if (i >= oi_lo && i < oi_hi && j >= 0 && j < nr) {
-      const uint32_t zr = a.meta[(long long)i + (long long)pitch * j] & 15u;
+      const uint32_t zr = bl_meta(bl, wr, tz + HALO) & 15u;
       dvr = d1_sum(zr, &sF[1][wr][tz + HALO], 1) * mcol;
       dvz = d1_sum(zr, &sF[3][wr][tz + HALO], 1) * mcol;
     }
@@ -484,7 +510,7 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
       continue;
     }
     const long long gi = (long long)i + (long long)pitch * j;
-    const uint32_t meta = a.meta[gi];
+    const uint32_t meta = bl_meta(bl, wr, wz);
     const uint32_t zr = meta & 15u, rr = (meta >> 4) & 15u;
     const double mz = mcol, mz2 = mcol2;  // the column's metrics, loaded with the window
     const double mr = a.detadr[j], mr2 = a.ddetadetadr[j];
@@ -503,12 +529,13 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
 #pragma unroll
       for (int k = 0; k < 5; ++k) pin1[k] = src1[gi + k * fs];
     }
-    // the target where a boundary patch or a sponge needs it
+    // the target where a boundary patch or a sponge needs it (rows from g.jtop on may be in the
+    // top sponge: the test needs no load of r)
     const uint32_t sid = (meta >> 8) & 255u;
     const bool top = r >= a.pc.sponge_start_top;
     const bool inflow = j >= 1 && zc <= a.pc.sponge_end_left;
     double qt[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (sid != 0 || top || inflow || (FORCE && (a.forcing & AJET_FORCING_FEEDBACK))) {
+    if (sid != 0 || j >= g.jtop || inflow || (FORCE && (a.forcing & AJET_FORCING_FEEDBACK))) {
 #pragma unroll
       for (int f = 0; f < 5; ++f) qt[f] = a.tgt[gi + f * fs];
     }
@@ -601,9 +628,9 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int t, bool write
     if (sid != 0 || top || inflow) {
       if (sid != 0) {
         for (int s = 0; s < SEQ_LEN; ++s) {
-          const int pidx = a.seq[sid * SEQ_LEN + s];
+          const int pidx = bl_seq(bl, sid * SEQ_LEN + s);
           if (pidx == 0) break;
-          const PatchDev P = a.patches[pidx - 1];
+          const PatchDev P = bl_patch(bl, pidx - 1);
           apply_patch(P.bc, P.dir, q, qt, d, mz, mr, a.pc.gam, rhs);
         }
       }
@@ -693,9 +720,10 @@ constexpr int stage_waves() {
 template <int STAGE, int MODE, bool INT, bool FORCE>
 __global__ __launch_bounds__(STAGE_THREADS, (stage_waves<STAGE, MODE, INT, FORCE>())) void stage_kernel(StageArgs a) {
   AJET_STAGE_LDS(MODE);
+  __shared__ StageBndLds<INT> bl;
   const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   if (li >= a.ntiles) return;
-  stage_body<STAGE, MODE, INT, FORCE>(a, a.tiles[li], li == 0, a.dt_from_dmax, a.parity, sF, sD, red);
+  stage_body<STAGE, MODE, INT, FORCE>(a, a.boxes[li], li == 0, a.dt_from_dmax, a.parity, sF, sD, red, bl);
 }
 
 // Interior and boundary tiles in one launch (list entry t: interior tile t, ~t: boundary tile
@@ -713,9 +741,12 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_mixed_kernel(StageArgs
   const StageArgs& a = *(const StageArgs*)__builtin_amdgcn_kernarg_segment_ptr();
   const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   if (li >= a.ntiles) return;
+  __shared__ BndLds bl;
+  NoLds nl;
   const int e = a.tiles[li];
-  if (e >= 0) stage_body<STAGE, MODE, true, FORCE>(a, e, false, a.dt_from_dmax, a.parity, sF, sD, red);
-  else stage_body<STAGE, MODE, false, FORCE>(a, ~e, li == 0, a.dt_from_dmax, a.parity, sF, sD, red);
+  const int4 box = a.boxes[li];
+  if (e >= 0) stage_body<STAGE, MODE, true, FORCE>(a, box, false, a.dt_from_dmax, a.parity, sF, sD, red, nl);
+  else stage_body<STAGE, MODE, false, FORCE>(a, box, li == 0, a.dt_from_dmax, a.parity, sF, sD, red, bl);
 }
 
 // The batched form: the work list holds (run, tile) entries of many runs, each run described by
@@ -724,11 +755,14 @@ template <int STAGE, int MODE, bool INT, bool FORCE>
 __global__ __launch_bounds__(STAGE_THREADS, INT ? STAGE_WAVES_PER_EU : 2) void stage_batch_kernel(
     const StageArgs* __restrict__ runs, const uint32_t* __restrict__ list, int n, int dt_from_dmax, int parity) {
   AJET_STAGE_LDS(MODE);
+  __shared__ StageBndLds<INT> bl;
   const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   if (li >= n) return;
   const uint32_t e = list[li];
-  stage_body<STAGE, MODE, INT, FORCE>(runs[(e >> 20) & 0x7ffu], (int)(e & 0xfffffu), (e >> 31) != 0u, dt_from_dmax,
-                                      parity, sF, sD, red);
+  const StageArgs& a = runs[(e >> 20) & 0x7ffu];
+  const int t = (int)(e & 0xfffffu), tzi = t % a.g.ntz, tri = t / a.g.ntz;
+  const int4 box = make_int4(a.zcuts[tzi], a.zcuts[tzi + 1], a.rcuts[tri], a.rcuts[tri + 1]);
+  stage_body<STAGE, MODE, INT, FORCE>(a, box, (e >> 31) != 0u, dt_from_dmax, parity, sF, sD, red, bl);
 }
 
 // ComputeTimeStep's sweep (ModInputOutput.f90:395-416) over the current state with the
