@@ -603,18 +603,19 @@ static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
     return e;
   }
   if (stage_mixed_supported(stage, mode)) {
-    // one launch for both tile classes: an event between two streams costs ~6 us of idle GPU
-    // each way, and the stage kernel holds 2 waves per SIMD here for either class
+    // one launch for both tile classes: the boundary tiles alone leave most of the GPU idle for
+    // ~30 us, and an event between two streams costs ~6 us of idle GPU each way (plain mode at
+    // 4400 x 640: stages 1-3 115.6 -> 105.7, 127 -> 119, 141 -> 133 us)
     a.tiles = c->mix_tiles;
     a.boxes = c->mix_box;
     a.ntiles = c->n_mix;
     return launch_stage_mixed(stage, mode, a, c->stream);
   }
   if (stage <= 2) {
-    // the interior tiles, then the boundary tiles, on the one stream: the boundary launch alone
-    // takes less than the two events between streams leave the GPU idle (stage 1 128 -> 116 us,
-    // stage 2 141 -> 128 us at 4400 x 640); stage 4, whose boundary tiles take longer, keeps
-    // them beside the interior tiles on the second stream
+    // the interior tiles, then the boundary tiles, on the one stream (the other physics modes'
+    // stages 1-2): the boundary launch alone takes less than the two events between streams
+    // leave the GPU idle; stage 4, whose boundary tiles take longer, keeps them beside the
+    // interior tiles on the second stream
     a.tiles = c->int_tiles;
     a.boxes = c->int_box;
     a.ntiles = c->n_int;

@@ -199,8 +199,8 @@ enum { MODE_PLAIN = 0, MODE_SC = 1, MODE_LAD = 2, MODE_INVISCID = 3 };
 
 hipError_t launch_stage(int stage, int mode, bool interior, const StageArgs& a, hipStream_t s);
 // A stage with the interior and boundary tiles in one launch: a.tiles holds the list of
-// mixed_tile_list (t interior, ~t boundary, a boundary tile first).  Offered where both classes
-// hold 2 waves per SIMD: stages 1-3 of the LAD mode, stage 3 of the plain mode.
+// mixed_tile_list (t interior, ~t boundary, a boundary tile first).  Stages 1-3 of the plain
+// mode (three blocks per CU) and of the LAD mode (two).
 bool stage_mixed_supported(int stage, int mode);
 hipError_t launch_stage_mixed(int stage, int mode, const StageArgs& a, hipStream_t s);
 // Batched stages (MODE_PLAIN and MODE_INVISCID): `runs` and `list` in device memory, list

@@ -91,9 +91,9 @@ constexpr int stage_nf() { return MODE == MODE_LAD ? 9 : 7; }
 // per-point step waits on a dependent load: the window's metadata words, the ordered patch lists
 // and the patches.  Interior tiles need none of it.
 struct BndLds {
-  uint32_t meta[WR][WZ];
+  uint16_t meta[WR][WZ];                // the metadata words (16 bits carry them)
   uint32_t seq[MAX_SEQ * SEQ_LEN / 4];  // the bytes of the patch lists, four to a word
-  PatchDev patches[MAX_PATCH];
+  int8_t bcdir[MAX_PATCH][2];           // each patch's bc type and direction
 };
 struct NoLds {
   char unused;
@@ -105,7 +105,12 @@ __device__ __forceinline__ uint32_t bl_meta(const BndLds& b, int wr, int wz) { r
 __device__ __forceinline__ uint32_t bl_meta(const NoLds&, int, int) { return ROW_INTERIOR | (ROW_INTERIOR << 4); }
 __device__ __forceinline__ int bl_seq(const BndLds& b, int k) { return reinterpret_cast<const uint8_t*>(b.seq)[k]; }
 __device__ __forceinline__ int bl_seq(const NoLds&, int) { return 0; }
-__device__ __forceinline__ PatchDev bl_patch(const BndLds& b, int k) { return b.patches[k]; }
+__device__ __forceinline__ PatchDev bl_patch(const BndLds& b, int k) {
+  PatchDev P{};
+  P.bc = b.bcdir[k][0];
+  P.dir = b.bcdir[k][1];
+  return P;
+}
 __device__ __forceinline__ PatchDev bl_patch(const NoLds&, int) { return PatchDev{}; }
 
 // Interior tiles address fields through raw buffer descriptors: one 32-bit byte offset per
@@ -401,18 +406,24 @@ __device__ __forceinline__ void stage_body(const StageArgs& a, int4 box, bool wr
         wlad[u][0] = a.lad[gi];
         wlad[u][1] = a.lad[gi + 2 * fs];
       }
-      if (k < WZ * WR) bl.meta[wr][wz] = inside[u] ? a.meta[gi] : (ROW_BLANK | (ROW_BLANK << 4));
+      if (k < WZ * WR) bl.meta[wr][wz] = (uint16_t)(inside[u] ? a.meta[gi] : (ROW_BLANK | (ROW_BLANK << 4)));
     }
   }
   if constexpr (!INT) {
-    // the patch lists and the patches (every list, every patch slot)
-    constexpr int NSW = MAX_SEQ * SEQ_LEN / 4, NPW = MAX_PATCH * (int)(sizeof(PatchDev) / 4);
-    static_assert(NSW + NPW <= 2 * STAGE_THREADS, "two words per thread");
+    // the patch lists and each patch's type and direction (every list, every patch slot)
+    constexpr int NSW = MAX_SEQ * SEQ_LEN / 4;
+    static_assert(NSW + MAX_PATCH <= 2 * STAGE_THREADS, "two entries per thread");
+    static_assert(AJET_BC_SPONGE < 128, "bc types fit a byte");
 #pragma unroll
     for (int u = 0; u < 2; ++u) {
       const int k = threadIdx.x + u * STAGE_THREADS;
-      if (k < NSW) bl.seq[k] = reinterpret_cast<const uint32_t*>(a.seq)[k];
-      else if (k < NSW + NPW) reinterpret_cast<int32_t*>(bl.patches)[k - NSW] = reinterpret_cast<const int32_t*>(a.patches)[k - NSW];
+      if (k < NSW) {
+        bl.seq[k] = reinterpret_cast<const uint32_t*>(a.seq)[k];
+      } else if (k < NSW + MAX_PATCH) {
+        const PatchDev P = a.patches[k - NSW];
+        bl.bcdir[k - NSW][0] = (int8_t)P.bc;
+        bl.bcdir[k - NSW][1] = (int8_t)P.dir;
+      }
     }
   }
 #pragma unroll
@@ -727,17 +738,15 @@ __global__ __launch_bounds__(STAGE_THREADS, (stage_waves<STAGE, MODE, INT, FORCE
 }
 
 // Interior and boundary tiles in one launch (list entry t: interior tile t, ~t: boundary tile
-// t; entry 0 is a boundary tile and writes the step scalars).  Used where both classes hold the
-// same waves per SIMD (the LDS of the LAD mode allows 2), so that the launch replaces two
-// launches on two streams and the events between them.  Stages 1-3 only: the stage-4 form ran
-// 350 us against 194 us for its interior tiles alone (its code, 47 KB, is the largest).
+// t; entry 0 is a boundary tile and writes the step scalars), so that the boundary tiles, which
+// alone leave most of the GPU idle for ~30 us, run beside the interior tiles.  Each XCD's range
+// starts with its boundary tiles (mixed_tile_list).
 template <int STAGE, int MODE, bool FORCE>
-__global__ __launch_bounds__(STAGE_THREADS, 2) void stage_mixed_kernel(StageArgs args) {
+__device__ __forceinline__ void mixed_body() {
   AJET_STAGE_LDS(MODE);
   // The arguments are read through the kernel-argument segment where each class uses them:
   // used as a by-value parameter they would all be loaded at the kernel's entry and, live across
   // the branch, spill scalar registers.
-  (void)args;
   const StageArgs& a = *(const StageArgs*)__builtin_amdgcn_kernarg_segment_ptr();
   const int li = xcd_tile((int)blockIdx.x, (int)gridDim.x);
   if (li >= a.ntiles) return;
@@ -747,6 +756,26 @@ __global__ __launch_bounds__(STAGE_THREADS, 2) void stage_mixed_kernel(StageArgs
   const int4 box = a.boxes[li];
   if (e >= 0) stage_body<STAGE, MODE, true, FORCE>(a, box, false, a.dt_from_dmax, a.parity, sF, sD, red, nl);
   else stage_body<STAGE, MODE, false, FORCE>(a, box, li == 0, a.dt_from_dmax, a.parity, sF, sD, red, bl);
+}
+
+// The LAD mode's stages 1-3: its 62 KB window allows two blocks per CU, at which both classes fit
+// their registers.
+template <int STAGE, int MODE, bool FORCE>
+__global__ __launch_bounds__(STAGE_THREADS, 2) void stage_mixed_kernel(StageArgs args) {
+  (void)args;
+  mixed_body<STAGE, MODE, FORCE>();
+}
+
+// The plain mode's stages 1-3 at three blocks per CU (50.5 + 2.2 KB of LDS each): the interior
+// tiles fit 168 VGPRs, the boundary tiles spill, which the interior tiles around them hide.  The
+// VGPR count is set explicitly: from the launch bound alone the compiler takes the boundary
+// class's 200-240 VGPRs and two blocks per CU.  (Stage 4 in this form ran 2.5x slower than its
+// two classes in two launches, for a reason not found; it keeps them apart.)
+template <int STAGE, int MODE, bool FORCE>
+__global__ __launch_bounds__(STAGE_THREADS, 3) __attribute__((amdgpu_num_vgpr(168))) void stage_mixed3_kernel(
+    StageArgs args) {
+  (void)args;
+  mixed_body<STAGE, MODE, FORCE>();
 }
 
 // The batched form: the work list holds (run, tile) entries of many runs, each run described by
@@ -855,7 +884,11 @@ template <bool FORCE>
 static void launch_mixed_t(int stage, int mode, dim3 grid, const StageArgs& a, hipStream_t s) {
   const dim3 block(STAGE_THREADS);
   if (mode == MODE_PLAIN) {
-    hipLaunchKernelGGL((stage_mixed_kernel<3, MODE_PLAIN, FORCE>), grid, block, 0, s, a);
+    switch (stage) {
+      case 1: hipLaunchKernelGGL((stage_mixed3_kernel<1, MODE_PLAIN, FORCE>), grid, block, 0, s, a); break;
+      case 2: hipLaunchKernelGGL((stage_mixed3_kernel<2, MODE_PLAIN, FORCE>), grid, block, 0, s, a); break;
+      default: hipLaunchKernelGGL((stage_mixed3_kernel<3, MODE_PLAIN, FORCE>), grid, block, 0, s, a); break;
+    }
     return;
   }
   switch (stage) {
@@ -866,7 +899,7 @@ static void launch_mixed_t(int stage, int mode, dim3 grid, const StageArgs& a, h
 }
 
 bool stage_mixed_supported(int stage, int mode) {
-  return (mode == MODE_LAD && stage >= 1 && stage <= 3) || (mode == MODE_PLAIN && stage == 3);
+  return (mode == MODE_LAD || mode == MODE_PLAIN) && stage >= 1 && stage <= 3;
 }
 
 hipError_t launch_stage_mixed(int stage, int mode, const StageArgs& a, hipStream_t s) {
