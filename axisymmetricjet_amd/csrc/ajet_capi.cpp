@@ -70,6 +70,7 @@ struct ajet_ctx {
   uint32_t gen = 0;                        // stamp of the sensor's flags (one per step)
   bool sensor_ready = false;               // the sensor of the current state ran after the last step
   bool skip_lad = false;                   // this step flagged no block: no LAD launches
+  bool last_skip = false;                  // the last step flagged no block
   uint32_t* any_dev = nullptr;             // the sensor's stamp when it flagged a block
   uint32_t* sensor_host = nullptr;         // pinned, fine-grained: [0] stamp, [1] any block flagged
   uint32_t* sensor_host_dev = nullptr;     // ... its device address
@@ -356,6 +357,7 @@ int ajet_set_state(ajet_ctx* c, const double* pv, const double* pv_target) {
   if (pv_target) copy_state_in(c, c->tgt, pv_target);
   c->learned_steps = 0;
   c->sensor_ready = false;
+  c->last_skip = false;
   StepScalars z{};
   HIPCHK(c, hipMemcpyAsync(c->sc, &z, sizeof z, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -542,48 +544,63 @@ static hipError_t learned_after_step(ajet_ctx* c, hipStream_t s) {
   return e;
 }
 
-static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStream_t s) {
+// Learned mode, stage 1, everything before the sensor's verdict: the network at a refresh, the
+// sensor of this state if it did not run after the last step, and the network's values put back
+// in the blocks the reference's LAD wrote last step.
+static hipError_t learned_stage1_pre(ajet_ctx* c, const double* q, hipStream_t s) {
   hipError_t e;
-  if (stage == 1) {
-    if (c->learned_steps % c->refresh == 0) {
-      // the network: lad_net and lad
-      if ((e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, s)) != hipSuccess) return e;
-      c->lad_full_restore = c->lad_restore = false;
-    } else if (c->lad_full_restore) {
-      if ((e = hipMemcpyAsync(c->lad, c->lad_net, sizeof(double) * 3 * (size_t)c->g.fstride, hipMemcpyDeviceToDevice,
-                              s)) != hipSuccess)
-        return e;
-      c->lad_full_restore = c->lad_restore = false;
-    }
+  if (c->learned_steps % c->refresh == 0) {
+    // the network: lad_net and lad
+    if ((e = launch_mlp_infer(mlp_args(c, q), c->infer_blocks, s)) != hipSuccess) return e;
+    c->lad_full_restore = c->lad_restore = false;
+  } else if (c->lad_full_restore) {
+    if ((e = hipMemcpyAsync(c->lad, c->lad_net, sizeof(double) * 3 * (size_t)c->g.fstride, hipMemcpyDeviceToDevice,
+                            s)) != hipSuccess)
+      return e;
+    c->lad_full_restore = c->lad_restore = false;
   }
-  if (stage == 4) ++c->learned_steps;
-  bool any = false;
-  if (stage == 1) {
-    // the sensor of this state: taken after the previous step (wait for its result), or now
-    if (!c->sensor_ready && (e = learned_sensor(c, q, s)) != hipSuccess) return e;
-    c->sensor_ready = false;
-    if (c->lad_restore) {
-      // blocks the reference's LAD wrote in the previous step and the sensor no longer flags
-      // get the network's values back (blocks flagged again are rewritten below)
-      LadArgs l = lad_args(c, q);
-      if ((e = launch_lad_restore(l, c->zr_flags, c->gen_written, c->lad_net, c->lad, s)) != hipSuccess) return e;
-      c->lad_restore = false;
-    }
-    if ((e = wait_sensor(c, &any)) != hipSuccess) return e;
-    c->skip_lad = !any;
+  // the sensor of this state: taken after the previous step (wait for its result), or now
+  if (!c->sensor_ready && (e = learned_sensor(c, q, s)) != hipSuccess) return e;
+  c->sensor_ready = false;
+  if (c->lad_restore) {
+    // blocks the reference's LAD wrote in the previous step and the sensor no longer flags
+    // get the network's values back (blocks flagged again are rewritten below)
+    LadArgs l = lad_args(c, q);
+    if ((e = launch_lad_restore(l, c->zr_flags, c->gen_written, c->lad_net, c->lad, s)) != hipSuccess) return e;
+    c->lad_restore = false;
   }
-  if (c->skip_lad) return hipSuccess;  // no block flagged: the LAD launches would all return at once
+  return hipSuccess;
+}
+
+// The reference's LAD on the blocks this step's sensor flagged (new_step: derive the tile masks
+// of the flagged blocks first, once per step).
+static hipError_t learned_fallback(ajet_ctx* c, const double* q, bool new_step, hipStream_t s) {
+  hipError_t e;
   LadArgs l = lad_args(c, q);
   l.nseg = c->nseg;
   l.gen = c->gen;  // the stamp of this step's sensor
   // the tile masks of the flagged blocks, for the four stages of this step
-  if (any && (e = launch_lad_masks(l, c->zr_flags, c->src_mask, c->str_mask, s)) != hipSuccess) return e;
+  if (new_step && (e = launch_lad_masks(l, c->zr_flags, c->src_mask, c->str_mask, s)) != hipSuccess) return e;
   l.zr_flags = c->zr_flags;
   l.src_mask = c->src_mask;
   l.str_mask = c->str_mask;
   c->lad_restore = true;
   c->gen_written = c->gen;
   return launch_lad(l, s);
+}
+
+static hipError_t launch_learned(ajet_ctx* c, int stage, const double* q, hipStream_t s) {
+  hipError_t e;
+  if (stage == 4) ++c->learned_steps;
+  bool any = false;
+  if (stage == 1) {
+    if ((e = learned_stage1_pre(c, q, s)) != hipSuccess) return e;
+    if ((e = wait_sensor(c, &any)) != hipSuccess) return e;
+    c->skip_lad = !any;
+    c->last_skip = !any;
+  }
+  if (c->skip_lad) return hipSuccess;  // no block flagged: the LAD launches would all return at once
+  return learned_fallback(c, q, any, s);
 }
 
 // The artificial coefficients of one stage of a run in LAD mode (the reference's or learned).
@@ -594,6 +611,26 @@ static hipError_t launch_coefficients(ajet_ctx* c, int stage, const double* q, h
 static hipError_t launch_stages(ajet_ctx* c, int stage, StageArgs a) {
   const int mode = physics_mode(c->p);
   hipError_t e;
+  if (mode == MODE_LAD && c->learned && stage == 1 && c->last_skip && c->sensor_ready && stage_mixed_supported(1, mode)) {
+    // Learned mode, the previous step flagged no block: stage 1 is launched at once on the guess
+    // that this step flags none either, and the host waits for the sensor's verdict while the GPU
+    // runs it (waiting first left the GPU idle ~22 us per step).  A step that flags blocks runs
+    // their reference LAD and stage 1 again; the first launch has logged dt and left it in the
+    // step scalars, which the second reads.
+    if ((e = learned_stage1_pre(c, a.qin, c->stream)) != hipSuccess) return e;
+    a.tiles = c->mix_tiles;
+    a.boxes = c->mix_box;
+    a.ntiles = c->n_mix;
+    if ((e = launch_stage_mixed(stage, mode, a, c->stream)) != hipSuccess) return e;
+    bool any = false;
+    if ((e = wait_sensor(c, &any)) != hipSuccess) return e;
+    c->skip_lad = !any;
+    c->last_skip = !any;
+    if (!any) return hipSuccess;
+    if ((e = learned_fallback(c, a.qin, true, c->stream)) != hipSuccess) return e;
+    a.dt_from_dmax = 0;
+    return launch_stage_mixed(stage, mode, a, c->stream);
+  }
   if (mode == MODE_LAD && stage == 0) {
     // the test hook's RHS takes the reference's coefficients of its input; in learned mode that
     // overwrites lad, which the next step restores from the network's values in lad_net

@@ -255,6 +255,9 @@ def test_unflagged_blocks_return_to_the_network_values():
         for k, s in enumerate(states):
             c1.run(1)
             assert np.array_equal(c1.state(), s), f"step {k + 1}"
+        # one time step logged per step, also where stage 1 ran twice (a step that flags blocks
+        # after one that flagged none launches stage 1 before the sensor's verdict, then again)
+        assert len(c3.dt_log()[0]) == 16 and len(c1.dt_log()[0]) == 16
         break
     assert found, "no threshold made the sensor let go of a block"
 
