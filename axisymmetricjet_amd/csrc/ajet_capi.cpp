@@ -39,8 +39,10 @@ struct ajet_ctx {
   int* mix_tiles = nullptr;  // both in one list (mixed_tile_list), for the LAD-mode stage kernel
   int n_mix = 0;
   int4 *int_box = nullptr, *bnd_box = nullptr, *mix_box = nullptr;  // the three lists' tile bounds
-  int* lad_tiles = nullptr;  // the strain and source kernels' tiles, split the same way
+  int* lad_tiles = nullptr;  // the source kernel's tiles: fused, or reading the strain kernel's values
   int n_lad_tiles = 0;
+  int* lad_str_tiles = nullptr;  // the strain kernel's tiles: interior or boundary
+  int n_lad_str = 0;
   int* lad_groups = nullptr;                       // line groups of the fused z/r filter
   int n_lad_groups = 0;
   int lad_seg_first = 0, lad_src_imin = -1, lad_str_imin = -1;  // zero-ramp work skipped
@@ -154,7 +156,8 @@ int ajet_destroy(ajet_ctx* c) {
   void* bufs[] = {c->Q0, c->Q1, c->Q2, c->A, c->S, c->tgt, c->lad, c->ladA, c->ladB, c->dzl, c->drl, c->ramp,
                   c->meta, c->zc, c->rc, c->dxidz,
                   c->ddxidxidz, c->detadr, c->ddetadetadr, c->patches, c->seq, c->zcuts, c->rcuts,
-                  c->int_tiles, c->bnd_tiles, c->mix_tiles, c->lad_tiles, c->lad_groups, c->codes, c->sc, c->dtlog,
+                  c->int_tiles, c->bnd_tiles, c->mix_tiles, c->lad_tiles, c->lad_str_tiles, c->lad_groups,
+                  c->codes, c->sc, c->dtlog,
                   c->mlpW, c->mlpB, c->mlpShift, c->mlpScale, c->zr_flags, c->src_mask, c->str_mask,
                   c->line_group, c->feat, c->feat_pts, c->probe_pts, c->probe_out, c->any_dev, c->lad_net, c->rinv,
                   c->int_box, c->bnd_box, c->mix_box};
@@ -318,12 +321,19 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
     HIPCHK(c, boxes(it, &c->int_box));
     HIPCHK(c, boxes(bt, &c->bnd_box));
     HIPCHK(c, boxes(mix, &c->mix_box));
-    classify_lad_tiles(c->s, nz, nr, it, bt);
+    std::vector<int> si, sb;
+    classify_lad_source(c->s, nz, nr, it, bt, si, sb);
+    auto upload = [&](const std::vector<int>& list, int** dst, int* n) {
+      *n = (int)list.size();
+      hipError_t e = hipMalloc((void**)dst, sizeof(int) * (list.size() + 1));
+      if (e == hipSuccess && !list.empty())
+        e = hipMemcpy(*dst, list.data(), sizeof(int) * list.size(), hipMemcpyHostToDevice);
+      return e;
+    };
     mixed_tile_list(it, bt, mix);
-    c->n_lad_tiles = (int)mix.size();
-    HIPCHK(c, hipMalloc((void**)&c->lad_tiles, sizeof(int) * (mix.size() + 1)));
-    if (!mix.empty())
-      HIPCHK(c, hipMemcpy(c->lad_tiles, mix.data(), sizeof(int) * mix.size(), hipMemcpyHostToDevice));
+    HIPCHK(c, upload(mix, &c->lad_tiles, &c->n_lad_tiles));
+    mixed_tile_list(si, sb, mix);
+    HIPCHK(c, upload(mix, &c->lad_str_tiles, &c->n_lad_str));
     std::vector<int> gr;
     build_lad_line_groups(c->s, nz, nr, LAD_LINES, gr);
     c->n_lad_groups = (int)gr.size() / 3;
@@ -453,6 +463,8 @@ static LadArgs lad_args(ajet_ctx* c, const double* q) {
   l.rcuts = c->rcuts;
   l.tiles = c->lad_tiles;
   l.ntiles = c->n_lad_tiles;
+  l.str_tiles = c->lad_str_tiles;
+  l.nstr = c->n_lad_str;
   l.rgroups = c->lad_groups;
   l.nrgroups = c->n_lad_groups;
   l.seg_first = c->lad_seg_first;

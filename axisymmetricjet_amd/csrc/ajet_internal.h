@@ -29,6 +29,7 @@ constexpr int WR = TR + 2 * HALO;
 constexpr int STAGE_THREADS = AJET_THREADS;
 
 constexpr int LAD_LINES = 64;  // r-lines one wave of the fused z/r filter walks
+constexpr int LAD_REACH = 7;   // points beyond a source tile at which it reads |S| and dil
 
 constexpr int MAX_PATCH = 64;
 constexpr int MAX_SEQ = 64;       // distinct ordered lists of boundary patches met at one point
@@ -91,6 +92,8 @@ int build_setup(const ajet_params& p, const double* z, const double* r, const in
 int build_tiling(Setup& s, int nz, int nr, std::string& err);
 void classify_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, std::vector<int>& boundary);
 void classify_lad_tiles(const Setup& s, int nz, int nr, std::vector<int>& interior, std::vector<int>& boundary);
+void classify_lad_source(const Setup& s, int nz, int nr, std::vector<int>& fused, std::vector<int>& other,
+                         std::vector<int>& str_interior, std::vector<int>& str_boundary);
 void mixed_tile_list(const std::vector<int>& interior, const std::vector<int>& boundary, std::vector<int>& out);
 void build_lad_line_groups(const Setup& s, int nz, int nr, int lines, std::vector<int>& groups);
 void lad_tables(const Setup& s, int nz, int nr, std::vector<double>& dz, std::vector<double>& dr,

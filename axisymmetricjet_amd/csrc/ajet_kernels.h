@@ -79,8 +79,11 @@ struct LadArgs {
   double* out;             // 3 fields: mu*, beta*, kappa*
   const int* zcuts;
   const int* rcuts;
-  const int* tiles;        // strain and source tiles: t interior (all points and stencil rows
-  int ntiles;              // interior), ~t boundary, in the order of mixed_tile_list
+  const int* tiles;        // source tiles: t fused (takes |S| and dil from the velocity), ~t
+  int ntiles;              // reads them from the strain kernel (classify_lad_source), in the
+                           // order of mixed_tile_list
+  const int* str_tiles;    // strain tiles, those the ~t source tiles read: t interior (all
+  int nstr;                // points and stencil rows interior), ~t boundary
   const int* rgroups;      // line groups of the fused z/r filter: (wb, out_lo, out_hi) each
   int nrgroups;
   Geometry g;

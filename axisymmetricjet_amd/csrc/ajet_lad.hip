@@ -1,8 +1,10 @@
 // gfx950 kernels of the localized artificial diffusivity (ComputeArtTransCoeffs,
 // ModShockCapturing.f90:5-106 with PerformTGfilter, ModTGfilter.f90:5-95):
 //
-//   lad_strain_kernel   |S|, dilatation and the Ducros-type switch f_sw at every point
-//   lad_source_kernel   D2(D2(.)) of |S|, dil and T in xi and in eta; the three quantities
+//   lad_strain_kernel   |S|, dilatation and the Ducros-type switch f_sw, on the tiles whose
+//                       values a source tile reads from memory (near closures and the axis)
+//   lad_source_kernel   D2(D2(.)) of |S|, dil and T in xi and in eta; the three quantities.
+//                       Interior tiles compute |S|, dil and f_sw themselves (fused tiles)
 //   lad_zrfilter_kernel the z filter, applied in place by the reference, i.e. the causal
 //                       recursion y_i = sum_{m<0} c_m y_{i+m} + sum_{m>=0} c_m x_{i+m}, then
 //                       the r filter, C Re (Pr) scaling and the tanh ramp -> mu*, beta*, kappa*
@@ -18,10 +20,37 @@
 namespace ajet {
 
 // ---- |S|, dil, f_sw (ComputeDilatation, ComputeMgnStrRate ModShockCapturing.f90:108-159, and
-// the switch of ComputeTobeFilteredQuantities :196-202).  Same tiles and 2-point window as the
-// stage kernel; the first derivatives take the patch closures and are zero at blanked points.
-// INT: tile t's points and stencil rows are all interior (see classify_lad_tiles), so no
-// metadata is read.
+// the switch of ComputeTobeFilteredQuantities :196-202) at one point from its four velocity
+// derivatives.  On the axis row (axis) Vr/r is replaced by dVr/dr (ModShockCapturing.f90:116-120,
+// 146-150).  The strain kernel and the fused source tiles both take this arithmetic.
+__device__ __forceinline__ void strain_point(double dVrdz, double dVzdz, double dVrdr, double dVzdr, double Vr,
+                                             double rcj, bool axis, double& S, double& dil, double& fsw) {
+  const double c13 = 1.0 / 3.0;
+  const double eps = (double)1e-15f;  // a default-REAL literal (ModShockCapturing.f90:202)
+  double S22;
+  if (axis) {
+    dil = 2.0 * dVrdr + dVzdz;
+    S22 = dVrdr - c13 * dil;
+  } else {
+    const double vbr = Vr / rcj;
+    dil = (dVrdr + vbr) + dVzdz;
+    S22 = vbr - c13 * dil;
+  }
+  const double S11 = dVrdr - c13 * dil;
+  const double S13 = 0.5 * (dVrdz + dVzdr);
+  const double S33 = dVzdz - c13 * dil;
+  // the nine squares summed in order; S12, S21, S23, S32 are zero and add exact zeros
+  S = sqrt((((S11 * S11 + S13 * S13) + S22 * S22) + S13 * S13) + S33 * S33);
+  const double vort = fabs(dVrdz - dVzdr);
+  const double md = -1.0 * dil;
+  const double H = md < 0.0 ? 0.0 : 1.0;
+  fsw = H * (dil * dil) / ((dil * dil + vort * vort) + eps);
+}
+
+// The strain kernel: the same tiles and 2-point window as the stage kernel; the first
+// derivatives take the patch closures and are zero at blanked points.  It covers the tiles
+// whose values the non-fused source tiles read (classify_lad_source).  INT: tile t's points
+// and stencil rows are all interior (classify_lad_tiles), so no metadata is read.
 template <bool INT>
 __device__ __forceinline__ void strain_tile(const LadArgs& a, int t, double (&sVr)[WR][WZ], double (&sVz)[WR][WZ]) {
   const Geometry& g = a.g;
@@ -67,8 +96,6 @@ __device__ __forceinline__ void strain_tile(const LadArgs& a, int t, double (&sV
   }
   __syncthreads();
   const int tz = threadIdx.x % TZ;
-  const double c13 = 1.0 / 3.0;
-  const double eps = (double)1e-15f;  // a default-REAL literal (ModShockCapturing.f90:202)
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
     const int ty = (int)threadIdx.x / TZ + p * (256 / TZ);
@@ -82,26 +109,8 @@ __device__ __forceinline__ void strain_tile(const LadArgs& a, int t, double (&sV
     const double dVzdz = d1_sum(zr, &sVz[wr][wz], 1) * mz;
     const double dVrdr = d1_sum(rr, &sVr[wr][wz], WZ) * mr[p];
     const double dVzdr = d1_sum(rr, &sVz[wr][wz], WZ) * mr[p];
-    const double Vr = sVr[wr][wz];
-    // the axis row takes Vr/r -> dVr/dr (ModShockCapturing.f90:116-120, 146-150)
-    double dil, S22;
-    if (!INT && j == 0) {
-      dil = 2.0 * dVrdr + dVzdz;
-      S22 = dVrdr - c13 * dil;
-    } else {
-      const double vbr = Vr / rcj[p];
-      dil = (dVrdr + vbr) + dVzdz;
-      S22 = vbr - c13 * dil;
-    }
-    const double S11 = dVrdr - c13 * dil;
-    const double S13 = 0.5 * (dVrdz + dVzdr);
-    const double S33 = dVzdz - c13 * dil;
-    // the nine squares summed in order; S12, S21, S23, S32 are zero and add exact zeros
-    const double S = sqrt((((S11 * S11 + S13 * S13) + S22 * S22) + S13 * S13) + S33 * S33);
-    const double vort = fabs(dVrdz - dVzdr);
-    const double md = -1.0 * dil;
-    const double H = md < 0.0 ? 0.0 : 1.0;
-    const double fsw = H * (dil * dil) / ((dil * dil + vort * vort) + eps);
+    double S, dil, fsw;
+    strain_point(dVrdz, dVzdz, dVrdr, dVzdr, sVr[wr][wz], rcj[p], !INT && j == 0, S, dil, fsw);
     a.A[gi] = S;
     a.A[gi + fs] = dil;
     a.A[gi + 2 * fs] = fsw;
@@ -114,8 +123,8 @@ __device__ __forceinline__ void strain_tile(const LadArgs& a, int t, double (&sV
 __global__ __launch_bounds__(256) void lad_strain_kernel(LadArgs a) {
   __shared__ double sVr[WR][WZ];
   __shared__ double sVz[WR][WZ];
-  for (int li = xcd_tile((int)blockIdx.x, (int)gridDim.x); li < a.ntiles; li += (int)gridDim.x) {
-    const int e = a.tiles[li];
+  for (int li = xcd_tile((int)blockIdx.x, (int)gridDim.x); li < a.nstr; li += (int)gridDim.x) {
+    const int e = a.str_tiles[li];
     if (e >= 0) strain_tile<true>(a, e, sVr, sVz);
     else strain_tile<false>(a, ~e, sVr, sVz);
   }
@@ -141,7 +150,6 @@ struct SourceLds {
   uint8_t cR[TR + 4][TZ];  // r row codes where the D2 along eta is taken
 };
 
-template <bool INT>
 __device__ __forceinline__ void source_tile(const LadArgs& a, int t, SourceLds& sh) {
   auto& sB = sh.sB;
   auto& sZ = sh.sZ;
@@ -157,7 +165,7 @@ __device__ __forceinline__ void source_tile(const LadArgs& a, int t, SourceLds& 
   const int tzi = t % g.ntz, tri = t / g.ntz;
   const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
   if (i1 <= a.src_imin) return;  // only feeds zero-ramp columns
-  if (!INT) {
+  {
     // codes of the points where a second derivative is taken; outside the grid it is zero,
     // which the blank code gives
     for (int k = threadIdx.x; k < TR * (TZ + 4); k += blockDim.x) {
@@ -195,32 +203,9 @@ __device__ __forceinline__ void source_tile(const LadArgs& a, int t, SourceLds& 
     // not all at once: the kernel's 72 VGPRs hold three loads in flight without spilling)
     constexpr int SW = 3;
     constexpr int NG = ((LWZ * LWR + 255) / 256 + SW - 1) / SW;
-    if constexpr (INT) {
-      // interior stencils reach 4 points from the tile, not LH = 7: only that part of the window
-      // is read (its points lie inside the grid; indices are clamped for tiles narrower than the
-      // window, whose extra points go unused)
-      constexpr int IB = LH - 4, IWZ = LWZ - 2 * IB, IWR = LWR - 2 * IB, NI = IWZ * IWR;
-      constexpr int NGI = ((NI + 255) / 256 + SW - 1) / SW;
-#pragma unroll 1
-      for (int gq = 0; gq < NGI; ++gq) {
-        double v[SW];
-        int dst[SW];
-#pragma unroll
-        for (int u = 0; u < SW; ++u) {
-          const int k = (int)threadIdx.x + 256 * (SW * gq + u);
-          const int iz = IB + k % IWZ, ir = IB + k / IWZ;
-          const int i = min(i0 - LH + iz, nz - 1), j = min(j0 - LH + ir, nr - 1);
-          dst[u] = k < NI ? ir * LWZ + iz : -1;
-          v[u] = src[k < NI ? (long long)i + (long long)pitch * j : (long long)i0 + (long long)pitch * j0];
-        }
-#pragma unroll
-        for (int u = 0; u < SW; ++u)
-          if (dst[u] >= 0) fB[dst[u]] = v[u];
-      }
-    }
     int wz = wz0, wr = wr0;
 #pragma unroll 1
-    for (int gq = 0; gq < (INT ? 0 : NG); ++gq) {
+    for (int gq = 0; gq < NG; ++gq) {
       double v[SW];
       bool vin[SW];
 #pragma unroll
@@ -248,14 +233,14 @@ __device__ __forceinline__ void source_tile(const LadArgs& a, int t, SourceLds& 
 #pragma unroll 1
     for (int k = threadIdx.x; k < TR * (TZ + 4); k += 256) {
       const int ty = k / (TZ + 4);
-      const uint32_t zr = INT ? ROW_INTERIOR : fcZ[k];
+      const uint32_t zr = fcZ[k];
       fZ[k] = d2_sum(zr, fB + k + (LWZ - TZ - 4) * ty + LH * LWZ + LH - 2, 1);
     }
     // sR[w][c] and cR[w][c] are element k = 32 w + c; its centre sB[w - 2 + LH][c + LH]
 #pragma unroll 1
     for (int k = threadIdx.x; k < (TR + 4) * TZ; k += 256) {
       const int w = k / TZ;
-      const uint32_t rr = INT ? ROW_INTERIOR : fcR[k];
+      const uint32_t rr = fcR[k];
       fR[k] = d2_sum(rr, fB + k + (LWZ - TZ) * w + (LH - 2) * LWZ + LH, LWZ);
     }
     __syncthreads();
@@ -263,7 +248,7 @@ __device__ __forceinline__ void source_tile(const LadArgs& a, int t, SourceLds& 
     for (int p = 0; p < NP; ++p) {
       const int ty = (int)threadIdx.x / TZ + p * ROWS;
       if (ty >= TR) continue;
-      const uint32_t zr = INT ? ROW_INTERIOR : cZ[ty][tz + 2], rr = INT ? ROW_INTERIOR : cR[ty + 2][tz];
+      const uint32_t zr = cZ[ty][tz + 2], rr = cR[ty + 2][tz];
       dz4[f][p] = d2_sum(zr, &sZ[ty][tz + 2], 1);
       dr4[f][p] = d2_sum(rr, &sR[ty + 2][tz], TZ);
       if (f == 2) Tc[p] = sB[ty + LH][tz + LH];
@@ -287,15 +272,206 @@ __device__ __forceinline__ void source_tile(const LadArgs& a, int t, SourceLds& 
   __syncthreads();  // the window and codes are reused by the next tile
 }
 
-// Interior and boundary tiles in one launch, as the strain kernel.
-// Seven blocks per CU: 22 KB of LDS allows them, and the bound holds the kernel to 72 VGPRs
-// (78 unbounded, which allows six), without spills.
-__global__ __launch_bounds__(256, 7) void lad_source_kernel(LadArgs a) {
-  __shared__ SourceLds sh;
+// ---- fused source tile: |S|, dil and f_sw from the velocity, then the three D2(D2(.)) pairs.
+// For a tile whose |S| points are interior, off the axis and inside the grid
+// (classify_lad_source), the strain kernel's interior arithmetic (strain_point) runs here, over
+// the tile widened by FH = 4 points on every side: the pairs read |S| and dil over the owned
+// rows at the owned columns +-4 and over the owned columns at the owned rows +-4 (the corners
+// are computed and not read).  The values equal the strain kernel's bit for bit and never pass
+// through memory.  The velocity window, 2 points wider still, is loaded in one round trip; T's
+// window is loaded during the strain arithmetic.
+constexpr int FH = 4;
+constexpr int FWZ = TZ + 2 * FH, FWR = TR + 2 * FH;  // the |S|, dil and T frame
+constexpr int VWZ = FWZ + 4, VWR = FWR + 4;          // the velocity window
+static_assert(FWZ + 2 * FWR <= 256, "one metric per thread");
+// Four blocks per CU: LDS would allow six, but the fused tile keeps 115 VGPRs (at the 96 that
+// five need it spills loop-invariant registers and runs 82 us per launch instead of 66).
+#ifndef AJET_SRC_BLOCKS
+#define AJET_SRC_BLOCKS 4
+#endif
+
+// The velocity is read (into registers) before |S| and dil are stored over it: 26 KB per block.
+struct FusedLds {
+  union {
+    struct {
+      double vr[VWR][VWZ];
+      double vz[VWR][VWZ];
+    } v;  // the strain arithmetic's velocity
+    struct {
+      double f[FWR][FWZ];    // |S|, then T
+      double dl[FWR][FWZ];   // dil
+      double z[TR][TZ + 4];  // a field's D2 along xi, columns i0-2 .. i0+TZ+1
+      double r[TR + 4][TZ];  // its D2 along eta, rows j0-2 .. j0+TR+1
+    } d;
+  } u;
+  double mz[FWZ], mr[FWR], rc[FWR];
+};
+
+__device__ __forceinline__ void source_tile_fused(const LadArgs& a, int t, FusedLds& sh) {
+  constexpr int ROWS = 256 / TZ;              // rows of owned points per pass of the block
+  constexpr int NP = (TR + ROWS - 1) / ROWS;  // owned points per thread
+  const Geometry& g = a.g;
+  const int nz = g.nz, nr = g.nr, pitch = g.pitch;
+  const long long fs = g.fstride;
+  if (a.src_mask && !a.src_mask[t]) return;
+  const int tzi = t % g.ntz, tri = t / g.ntz;
+  const int i0 = a.zcuts[tzi], i1 = a.zcuts[tzi + 1], j0 = a.rcuts[tri], j1 = a.rcuts[tri + 1];
+  if (i1 <= a.src_imin) return;  // only feeds zero-ramp columns
+  const int tid = (int)threadIdx.x, tz = tid % TZ;
+  // frame position (sz, sr) is the point (i0 - FH + sz, j0 - FH + sr); positions a narrow tile
+  // does not need are clamped into the grid and go unused
+  auto gidx = [&](int i, int j) { return (long long)min(max(i, 0), nz - 1) + (long long)pitch * min(max(j, 0), nr - 1); };
+  constexpr int NV = (VWZ * VWR + 255) / 256;
+  double vr[NV], vz[NV];
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int k = tid + 256 * u;
+    const long long gi = k < VWZ * VWR ? gidx(i0 - FH - 2 + k % VWZ, j0 - FH - 2 + k / VWZ) : gidx(i0, j0);
+    vr[u] = a.q[gi + fs];
+    vz[u] = a.q[gi + 3 * fs];
+  }
+  double m = 0.0;
+  if (tid < FWZ) m = a.dxidz[min(i0 - FH + tid, nz - 1)];
+  else if (tid < FWZ + FWR) m = a.detadr[min(j0 - FH + tid - FWZ, nr - 1)];
+  else if (tid < FWZ + 2 * FWR) m = a.rc[min(j0 - FH + tid - FWZ - FWR, nr - 1)];
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int k = tid + 256 * u;
+    if (k < VWZ * VWR) {
+      (&sh.u.v.vr[0][0])[k] = vr[u];
+      (&sh.u.v.vz[0][0])[k] = vz[u];
+    }
+  }
+  if (tid < FWZ) sh.mz[tid] = m;
+  else if (tid < FWZ + FWR) sh.mr[tid - FWZ] = m;
+  else if (tid < FWZ + 2 * FWR) sh.rc[tid - FWZ - FWR] = m;
+  __syncthreads();
+  auto strain_at = [&](int sz, int sr, double& S, double& dil, double& fsw) {
+    const double* pr = &sh.u.v.vr[sr + 2][sz + 2];
+    const double* pz = &sh.u.v.vz[sr + 2][sz + 2];
+    const double mz = sh.mz[sz], mr = sh.mr[sr];
+    const double dVrdz = d1_sum(ROW_INTERIOR, pr, 1) * mz;
+    const double dVzdz = d1_sum(ROW_INTERIOR, pz, 1) * mz;
+    const double dVrdr = d1_sum(ROW_INTERIOR, pr, VWZ) * mr;
+    const double dVzdr = d1_sum(ROW_INTERIOR, pz, VWZ) * mr;
+    strain_point(dVrdz, dVzdz, dVrdr, dVzdr, *pr, sh.rc[sr], false, S, dil, fsw);
+  };
+  // the owned points, then the ring around them: FH rows above and below the owned columns,
+  // FH columns left and right of the owned rows
+  constexpr int NRING = 2 * FH * TZ + 2 * FH * TR, NU = (NRING + 255) / 256;
+  double fsw[NP], Sv[NP + NU], Dv[NP + NU];
+  int at[NP + NU];  // frame element, -1: none
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int sz = tz + FH, sr = tid / TZ + p * ROWS + FH;
+    fsw[p] = 0.0;
+    at[p] = -1;
+    if (sr < TR + FH) {
+      strain_at(sz, sr, Sv[p], Dv[p], fsw[p]);
+      at[p] = sr * FWZ + sz;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int k = tid + 256 * u;
+    at[NP + u] = -1;
+    if (k >= NRING) continue;
+    int sz, sr;
+    if (k < 2 * FH * TZ) {
+      const int kk = k < FH * TZ ? k : k - FH * TZ;
+      sr = kk / TZ + (k < FH * TZ ? 0 : FH + TR);
+      sz = FH + kk % TZ;
+    } else {
+      const int kk = k - 2 * FH * TZ, side = kk < FH * TR ? 0 : 1, k2 = kk - side * FH * TR;
+      sr = FH + k2 / FH;
+      sz = k2 % FH + side * (FH + TZ);
+    }
+    double unused;
+    strain_at(sz, sr, Sv[NP + u], Dv[NP + u], unused);
+    at[NP + u] = sr * FWZ + sz;
+  }
+  __syncthreads();  // the velocity is read: |S| and dil take its place
+  // T's window and the owned points' pointwise inputs, in flight during the first pass
+  constexpr int NT = (FWZ * FWR + 255) / 256;
+  double tv[NT], rho[NP], dr[NP];
+#pragma unroll
+  for (int u = 0; u < NT; ++u) {
+    const int k = tid + 256 * u;
+    tv[u] = a.q[4 * fs + (k < FWZ * FWR ? gidx(i0 - FH + k % FWZ, j0 - FH + k / FWZ) : gidx(i0, j0))];
+  }
+  const double dz = a.dzl[min(i0 + tz, nz - 1)], dz2 = dz * dz;
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int ty = tid / TZ + p * ROWS;
+    rho[p] = a.q[gidx(i0 + tz, j0 + ty)];
+    dr[p] = a.drl[min(j0 + ty, nr - 1)];
+  }
+#pragma unroll
+  for (int u = 0; u < NP + NU; ++u)
+    if (at[u] >= 0) {
+      (&sh.u.d.f[0][0])[at[u]] = Sv[u];
+      (&sh.u.d.dl[0][0])[at[u]] = Dv[u];
+    }
+  __syncthreads();
+  // one field's D2 along xi and eta, then its second pair at the owned points, weighted by the
+  // grid spacings (squared for |S| and dil) and summed: the magnitudes ComputeTobeFilteredQuantities
+  // scales (ModShockCapturing.f90:204-222)
+  double mag[3][NP];
+  auto pass = [&](const double (*F)[FWZ], int fi) {
+    for (int k = tid; k < TR * (TZ + 4); k += 256) {
+      const int ty = k / (TZ + 4), c = k % (TZ + 4);
+      sh.u.d.z[ty][c] = d2_sum(ROW_INTERIOR, &F[ty + FH][c + 2], 1);
+    }
+    for (int k = tid; k < (TR + 4) * TZ; k += 256) {
+      const int w = k / TZ, c = k % TZ;
+      sh.u.d.r[w][c] = d2_sum(ROW_INTERIOR, &F[w + 2][c + FH], FWZ);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int ty = tid / TZ + p * ROWS;
+      if (ty >= TR) continue;
+      const double dz4 = d2_sum(ROW_INTERIOR, &sh.u.d.z[ty][tz + 2], 1);
+      const double dr4 = d2_sum(ROW_INTERIOR, &sh.u.d.r[ty + 2][tz], TZ);
+      mag[fi][p] = fi < 2 ? fabs(dz4 * dz2 + dr4 * (dr[p] * dr[p])) : fabs(dz4 * dz + dr4 * dr[p]);
+    }
+  };
+  pass(sh.u.d.f, 0);
+  // |S| has been read: T takes its place
+#pragma unroll
+  for (int u = 0; u < NT; ++u) {
+    const int k = tid + 256 * u;
+    if (k < FWZ * FWR) (&sh.u.d.f[0][0])[k] = tv[u];
+  }
+  __syncthreads();
+  pass(sh.u.d.dl, 1);
+  __syncthreads();
+  pass(sh.u.d.f, 2);
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    const int ty = tid / TZ + p * ROWS;
+    const int i = i0 + tz, j = j0 + ty;
+    if (ty >= TR || i >= i1 || j >= j1) continue;
+    const long long gi = (long long)i + (long long)pitch * j;
+    const double T = sh.u.d.f[ty + FH][tz + FH];
+    a.B[gi] = rho[p] * mag[0][p];
+    a.B[gi + fs] = rho[p] * fsw[p] * mag[1][p];
+    a.B[gi + 2 * fs] = rho[p] / (a.gam * sqrt(T)) * mag[2][p];
+  }
+  __syncthreads();  // the frame is reused by the next tile
+}
+
+// Fused and other tiles in one launch: list entry t >= 0 is fused tile t, ~t a tile that reads
+// the strain kernel's values.
+__global__ __launch_bounds__(256, AJET_SRC_BLOCKS) void lad_source_kernel(LadArgs a) {
+  __shared__ union {
+    SourceLds general;
+    FusedLds fused;
+  } sh;
   for (int li = xcd_tile((int)blockIdx.x, (int)gridDim.x); li < a.ntiles; li += (int)gridDim.x) {
     const int e = a.tiles[li];
-    if (e >= 0) source_tile<true>(a, e, sh);
-    else source_tile<false>(a, ~e, sh);
+    if (e >= 0) source_tile_fused(a, e, sh.fused);
+    else source_tile(a, ~e, sh.general);
   }
 }
 
@@ -830,10 +1006,8 @@ hipError_t launch_lad(const LadArgs& a, hipStream_t s) {
   auto grid = [&](int n) { return dim3((unsigned)(((masked && n > MASKED_GRID ? MASKED_GRID : n) + 7) / 8 * 8)); };
   // the interior and the boundary tiles of each kernel in one launch: a boundary tile's launch
   // of its own took 10-25 us, most of it latency with a few hundred blocks on 256 CUs
-  if (a.ntiles > 0) {
-    hipLaunchKernelGGL(lad_strain_kernel, grid(a.ntiles), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(lad_source_kernel, grid(a.ntiles), dim3(256), 0, s, a);
-  }
+  if (a.nstr > 0) hipLaunchKernelGGL(lad_strain_kernel, grid(a.nstr), dim3(256), 0, s, a);
+  if (a.ntiles > 0) hipLaunchKernelGGL(lad_source_kernel, grid(a.ntiles), dim3(256), 0, s, a);
   const int nseg = (g.nz + a.zseg - 1) / a.zseg;
   if (nseg > a.seg_first)
     hipLaunchKernelGGL(lad_zrfilter_kernel, dim3(nseg - a.seg_first, a.nrgroups, 3), dim3(2 * ZLINES), 0, s, a);

@@ -308,6 +308,59 @@ void classify_lad_tiles(const Setup& s, int nz, int nr, std::vector<int>& interi
     }
 }
 
+// Source tiles of the artificial-diffusivity kernels that take |S| and the dilatation from the
+// velocity themselves (fused), and the others, which read them from the strain kernel.  A tile
+// is fused when it is one of classify_lad_tiles' interior tiles and every point where its
+// second-derivative pairs read |S| and dil -- the owned rows over the owned columns +-4, the
+// owned columns over the owned rows +-4 -- takes the interior first derivative along z and
+// along r, lies off the axis, and has the velocity it differentiates (2 more points along z
+// and r) inside the grid.  strain: the tiles whose |S|, dil and f_sw a non-fused tile reads,
+// i.e. every tile within LAD_REACH points of one (its own included), split as
+// classify_lad_tiles splits them.
+void classify_lad_source(const Setup& s, int nz, int nr, std::vector<int>& fused, std::vector<int>& other,
+                         std::vector<int>& str_interior, std::vector<int>& str_boundary) {
+  std::vector<int> it, bt;
+  classify_lad_tiles(s, nz, nr, it, bt);
+  const int ntz = (int)s.zcuts.size() - 1, ntr = (int)s.rcuts.size() - 1;
+  std::vector<uint8_t> is_int((size_t)ntz * ntr, 0);
+  for (int t : it) is_int[(size_t)t] = 1;
+  auto interior_point = [&](int i, int j) {
+    const uint32_t m = s.meta[(size_t)i + (size_t)nz * j];
+    return (m & 15u) == ROW_INTERIOR && ((m >> 4) & 15u) == ROW_INTERIOR;
+  };
+  fused.clear();
+  other.clear();
+  for (int tr = 0; tr < ntr; ++tr)
+    for (int tz = 0; tz < ntz; ++tz) {
+      const int t = tr * ntz + tz;
+      const int i0 = s.zcuts[tz], i1 = s.zcuts[tz + 1], j0 = s.rcuts[tr], j1 = s.rcuts[tr + 1];
+      bool ok = is_int[(size_t)t] && i0 - 6 >= 0 && i1 + 6 <= nz && j0 - 6 >= 1 && j1 + 6 <= nr;
+      for (int j = j0; ok && j < j1; ++j)
+        for (int i = i0 - 4; ok && i < i1 + 4; ++i) ok = interior_point(i, j);
+      for (int j = j0 - 4; ok && j < j1 + 4; ++j)
+        for (int i = i0; ok && i < i1; ++i) ok = interior_point(i, j);
+      (ok ? fused : other).push_back(t);
+    }
+  // the tiles any non-fused tile's window (LAD_REACH beyond it) touches
+  std::vector<uint8_t> need((size_t)ntz * ntr, 0);
+  for (int t : other) {
+    const int tz = t % ntz, tr = t / ntz;
+    const int lo_i = s.zcuts[tz] - LAD_REACH, hi_i = s.zcuts[tz + 1] + LAD_REACH;
+    const int lo_j = s.rcuts[tr] - LAD_REACH, hi_j = s.rcuts[tr + 1] + LAD_REACH;
+    for (int ur = 0; ur < ntr; ++ur) {
+      if (s.rcuts[ur + 1] <= lo_j || s.rcuts[ur] >= hi_j) continue;
+      for (int uz = 0; uz < ntz; ++uz)
+        if (s.zcuts[uz + 1] > lo_i && s.zcuts[uz] < hi_i) need[(size_t)ur * ntz + uz] = 1;
+    }
+  }
+  str_interior.clear();
+  str_boundary.clear();
+  for (int t : it)
+    if (need[(size_t)t]) str_interior.push_back(t);
+  for (int t : bt)
+    if (need[(size_t)t]) str_boundary.push_back(t);
+}
+
 // One launch list for interior and boundary tiles: interior tile t as t, boundary tile t as ~t.
 // A launch of pad8(n) blocks gives XCD x (blocks x, x + 8, ...) the list positions
 // [x C, (x + 1) C), C = pad8(n) / 8, in order.  Each XCD's range starts with its share of the

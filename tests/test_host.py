@@ -112,6 +112,41 @@ def test_mixed_tile_list(tmp_path):
     assert out.returncode == 0 and out.stdout.strip() == "ok", out.stdout
 
 
+def _case_file(path, case):
+    lines = np.asarray(case.bc_lines, dtype=np.int64).reshape(-1, 7)
+    with open(path, "wb") as f:
+        f.write(np.array([case.nz, case.nr, len(lines)], dtype=np.int32).tobytes())
+        f.write(np.asarray(case.z, dtype=np.float64).tobytes(order="F"))
+        f.write(np.asarray(case.r, dtype=np.float64).tobytes(order="F"))
+        f.write(np.asarray(case.iblank, dtype=np.int32).tobytes(order="F"))
+        f.write(lines.tobytes())
+
+
+@pytest.mark.parametrize("name", ["jet440", "nozzle96", "jet4400", "nozzle4400"])
+def test_lad_source_tiles(tmp_path, name):
+    """Which LAD source tiles compute the strain rate and dilatation themselves (host C++, built
+    here with g++), checked point by point against the metadata: fused tiles read only interior
+    points off the axis, inside the grid; every point a non-fused tile reads is in a strain
+    tile.  On the plain jet every interior tile is fused; the nozzle's lip leaves some that are
+    not."""
+    case = {"jet440": lambda: jc.make_jet(440, 64, seed=1),
+            "nozzle96": lambda: jc.make_nozzle(96, 48, i_lip=24, j_in=16, j_out=22),
+            "jet4400": lambda: jc.make_jet(4400, 640, seed=1),
+            "nozzle4400": lambda: jc.make_nozzle(4400, 640, i_lip=597, j_in=150, j_out=169)}[name]()
+    exe, data = str(tmp_path / "lad_tiles_check"), str(tmp_path / "case.bin")
+    _case_file(data, case)
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                           os.path.join(ROOT, "tests", "lad_tiles_check.cpp"),
+                           os.path.join(ROOT, "axisymmetricjet_amd", "csrc", "ajet_setup.cpp"), "-o", exe])
+    out = subprocess.run([exe, data], capture_output=True, text=True)
+    assert out.returncode == 0 and out.stdout.startswith("ok"), out.stdout
+    fused, other, strain, total = (int(x) for x in re.findall(r"\d+", out.stdout))
+    assert fused + other == total and fused > 0
+    if name == "jet4400":
+        # the strain kernel keeps the tiles within reach of the edges' boundary tiles
+        assert strain < 0.3 * total and other < 0.2 * total
+
+
 def test_interior_fast_math(tmp_path):
     """The interior tiles' FP64 helpers (csrc/ajet_fastmath.h), built for the host with hipcc:
     T**0.666 within 1e-15 of the C library's pow over its whole fast range (2^-8, 2^8) and the
