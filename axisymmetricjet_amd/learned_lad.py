@@ -52,14 +52,15 @@ def snapshot_samples(ctx, q, zc, rng, n):
 
 
 def generate(machs=MACHS, nz=4400, nr=640, warm=200, every=150, nsnap=12, per_snap=8192, seed=7, attempts=4,
-             log=print):
+             log=print, snapshot_ids=False):
     """Samples from plain-LAD runs at each Mach number: nsnap snapshots every `every` steps
     after `warm` steps.  Some noise seeds make the reference's scheme blow up at the corner of
     the inflow and top boundaries (z = 0, r = r_max) within a few hundred steps; a run whose
     state turns non-finite or whose |Vr| passes 1 is dropped with its snapshots and the next
-    seed is tried."""
+    seed is tried.  snapshot_ids: also return each sample's snapshot, (Mach index, snapshot
+    index) as mach * nsnap + snapshot."""
     rng = np.random.default_rng(seed)
-    xs, ts = [], []
+    xs, ts, ids = [], [], []
     for k, mach in enumerate(machs):
         for a in range(attempts):
             s0 = seed + 101 * k + 7919 * a
@@ -83,9 +84,12 @@ def generate(machs=MACHS, nz=4400, nr=640, warm=200, every=150, nsnap=12, per_sn
             if ok:
                 xs += got_x
                 ts += got_t
+                ids += [np.full(len(gx), k * nsnap + n, np.int32) for n, gx in enumerate(got_x)]
                 log(f"samples: Mach {mach} (seed {s0}): {nsnap} snapshots to step {warm + nsnap * every}")
                 break
             log(f"samples: Mach {mach} (seed {s0}): the run left the stable range, dropped")
+    if snapshot_ids:
+        return np.concatenate(xs), np.concatenate(ts), np.concatenate(ids)
     return np.concatenate(xs), np.concatenate(ts)
 
 
@@ -257,6 +261,46 @@ def load(path=DEFAULT_WEIGHTS):
         return {k: np.array(f[k], np.float32) for k in FIELDS}
 
 
+def snapshot_study(counts=(5, 15, 60, 240, 960), nz=4400, nr=640, warm=200, every=10, per_snap=2048, seed=7,
+                   held_out_steps=1100, steps=(6000, 3000, 3000), batch=4096, equal_samples=122880, log=print):
+    """Held-out coefficient error against the number of training snapshots: one set of
+    max(counts) snapshots (max(counts) / 5 per Mach number, `every` steps apart, per_snap
+    stencils each), then networks trained with the standard schedule on each evenly spaced
+    subset of counts[k] snapshots (the same number from every Mach number) -- on all of the
+    subset's samples, and on a random equal_samples of them where it has more -- scored on the
+    benchmark's Mach 1.5 run (noise seed 1234, not a training seed) after held_out_steps."""
+    per_mach = max(counts) // len(MACHS)
+    t0 = time.time()
+    x, t, sid = generate(nz=nz, nr=nr, warm=warm, every=every, nsnap=per_mach, per_snap=per_snap, seed=seed,
+                         log=log, snapshot_ids=True)
+    log(f"study: {len(x)} samples from {len(np.unique(sid))} snapshots in {time.time() - t0:.1f} s")
+    case = jc.make_jet(nz, nr, seed=1234, mach=1.5)
+    ctx = capi.Context(case, **LAD)
+    ctx.run(held_out_steps)
+    q = ctx.state()
+    rng = np.random.default_rng(seed + 1)
+    rows = []
+    for n in counts:
+        k = n // len(MACHS)
+        pick = np.unique(((np.arange(k) + 0.5) * per_mach / k).astype(np.int32))  # centred in k equal parts
+        keep = np.flatnonzero(np.isin(sid % per_mach, pick))
+        subsets = [("all", keep)]
+        if len(keep) > equal_samples:
+            subsets.append(("equal", np.sort(rng.choice(keep, equal_samples, replace=False))))
+        for kind, idx in subsets:
+            t1 = time.time()
+            model, losses = train(x[idx], t[idx], steps=steps, batch=batch, seed=seed, log=lambda s_: None)
+            err = coefficient_errors(ctx, q, model)
+            rows.append({"snapshots": int(len(pick) * len(MACHS)), "samples": int(len(idx)), "kind": kind,
+                         "final_loss": float(losses[-50:].mean()), "err_beta": err["beta"],
+                         "err_kappa": err["kappa"], "train_s": time.time() - t1})
+            log(f"study: {rows[-1]['snapshots']} snapshots, {rows[-1]['samples']} samples: held-out beta* "
+                f"{err['beta']:.3f}, kappa* {err['kappa']:.3f}, final loss {rows[-1]['final_loss']:.4f}")
+    ctx.close()
+    return {"every": every, "warm": warm, "per_snap": per_snap, "held_out_steps": held_out_steps,
+            "steps": list(steps), "batch": batch, "equal_samples": equal_samples, "rows": rows}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
@@ -269,8 +313,18 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--steps", type=int, nargs=3, default=(6000, 3000, 3000))
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--study", default=None, metavar="JSON",
+                    help="instead of training the shipped weights: held-out error against snapshot count "
+                         "(snapshot_study), written to this file")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
+    if a.study:
+        import json
+        res = snapshot_study(nz=a.nz, nr=a.nr, seed=a.seed, batch=a.batch, steps=tuple(a.steps), log=log)
+        with open(a.study, "w") as f:
+            json.dump(res, f, indent=1)
+        log(f"wrote {a.study}")
+        return
     t0 = time.time()
     x, t = generate(nz=a.nz, nr=a.nr, warm=a.warm, every=a.every, nsnap=a.nsnap, per_snap=a.per_snap,
                     seed=a.seed, log=log)
