@@ -74,6 +74,8 @@ struct ajet_ctx {
   bool skip_lad = false;                   // this step flagged no block: no LAD launches
   bool last_skip = false;                  // the last step flagged no block
   uint32_t* any_dev = nullptr;             // the sensor's stamp when it flagged a block
+  int sensor_i0 = 0;                       // first column whose ramp exceeds ramp_min
+  std::vector<double> ramp_h;              // the ramp, on the host
   uint32_t* sensor_host = nullptr;         // pinned, fine-grained: [0] stamp, [1] any block flagged
   uint32_t* sensor_host_dev = nullptr;     // ... its device address
   hipEvent_t ev_sensor = nullptr;          // the sensor's result is published
@@ -262,6 +264,7 @@ int ajet_create(const ajet_params* p, const double* z, const double* r, const in
   }
   std::vector<double> dzl, drl, ramp;
   lad_tables(c->s, nz, nr, dzl, drl, ramp);
+  c->ramp_h = ramp;
   std::vector<double> rinv((size_t)nr);
   for (int j = 0; j < nr; ++j) rinv[(size_t)j] = c->s.rc[(size_t)j] > 0.0 ? 1.0 / c->s.rc[(size_t)j] : 0.0;
   struct V { double** d; const std::vector<double>* h; } vs[] = {
@@ -517,10 +520,9 @@ static hipError_t learned_sensor(ajet_ctx* c, const double* q, hipStream_t s) {
   LadArgs l = lad_args(c, q);
   l.nseg = c->nseg;
   l.gen = ++c->gen;
-  if ((e = launch_lad_sensor(l, c->zr_flags, c->any_dev, c->line_group, c->sensor_thr, c->ramp_min, s)) !=
-      hipSuccess)
+  if ((e = launch_lad_sensor(l, c->zr_flags, c->any_dev, c->line_group, c->sensor_thr, c->ramp_min, c->sensor_i0,
+                             c->sensor_host_dev, s)) != hipSuccess)
     return e;
-  if ((e = launch_lad_publish(c->any_dev, c->gen, c->sensor_host_dev, s)) != hipSuccess) return e;
   return hipEventRecord(c->ev_sensor, s);
 }
 
@@ -931,6 +933,13 @@ int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_thres
   c->out_scale[1] = model->out_scale[1];
   c->sensor_thr = sensor_threshold;
   c->ramp_min = ramp_min;
+  // the sensor starts at the first column whose ramp exceeds ramp_min (the ramp rises with z)
+  c->sensor_i0 = g.nz;
+  for (int i = 0; i < g.nz; ++i)
+    if (c->ramp_h[(size_t)i] > ramp_min) {
+      c->sensor_i0 = i;
+      break;
+    }
   c->refresh = refresh;
   c->learned = true;
   return AJET_OK;

@@ -114,12 +114,12 @@ int lad_zrfilter_slots(int device);
 int lad_zfilter_segment(const double* ramp, int nz, int ngroups, int slots);
 void lad_skip_bounds(const double* ramp, int nz, int seg, int& seg_first, int& src_imin, int& str_imin);
 // the dilatation sensor of the learned mode: flags[group(j) * nseg + i / seg] = gen where
-// max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min, and *any = gen when it flags a
-// block; then the publication of its result to host-visible memory, host[1] = (*any == gen) and
-// host[0] = gen (stored last, with system-scope release); the tile masks of the flagged blocks
+// max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min (columns ifirst on, where the ramp
+// exceeds ramp_min), and *any = gen when it flags a block; then the publication of its result
+// to host-visible memory, host[1] = (*any == gen) and host[0] = gen (stored last, with
+// system-scope release); the tile masks of the flagged blocks
 hipError_t launch_lad_sensor(const LadArgs& a, uint32_t* flags, uint32_t* any, const int* line_group, double thr,
-                             double ramp_min, hipStream_t s);
-hipError_t launch_lad_publish(const uint32_t* any, uint32_t gen, uint32_t* host, hipStream_t s);
+                             double ramp_min, int ifirst, uint32_t* host, hipStream_t s);
 hipError_t launch_lad_masks(const LadArgs& a, const uint32_t* flags, uint8_t* src_mask, uint8_t* str_mask,
                             hipStream_t s);
 // dst = src (3 fields) over the output points of every z/r filter block whose flag equals gen
