@@ -162,6 +162,9 @@ def main(argv=None):
                     help="steps of the scoring runs of the under-expanded jet, npr 1.5 (0: skip; finite to ~2200 "
                          "steps from its initial condition)")
     ap.add_argument("--shocked-warm", type=int, default=750)
+    ap.add_argument("--shocked-timing-steps", type=int, default=400,
+                    help="steps of the learned and the reference-LAD mode timed on the under-expanded jet (npr 1.5) "
+                         "from its state after --shocked-warm steps, where the sensor flags blocks (0: skip)")
     ap.add_argument("--combined-long-steps", type=int, default=0,
                     help="one long run of the combined batch (the configured 10000 steps): learned and reference-LAD "
                          "batches with probes every 8 steps, ms/step, the step each run leaves the finite range, "
@@ -314,6 +317,30 @@ def main(argv=None):
                                           "flop_per_point": MLP_FLOP_PER_POINT}}
         log(f"learned: {learned['ms_per_step']:.4f} ms/step, x{learned['speedup_vs_lad'] or 0:.3f} vs LAD, "
             f"inference {mlp_ms:.4f} ms ({tflops:.1f} TFLOP/s), fallback {frac:.4f}")
+        if args.shocked_timing_steps > 0:
+            # the speed-up where the sensor has work: the under-expanded jet from a developed state
+            sj = jc.make_jet(nz, nr, seed=1234, npr=1.5)
+            lad_kw = dict(shock_capturing=1, C_mu=jc.LAD_C[0], C_beta=jc.LAD_C[1], C_kappa=jc.LAD_C[2])
+            w = capi.Context(sj, **lad_kw)
+            w.run(args.shocked_warm)
+            start = w.state()
+            w.close()
+            sh = {"steps": args.shocked_timing_steps, "warm_steps": args.shocked_warm, "sensor_threshold": 0.05}
+            for mode in ("lad", "learned"):
+                c = capi.Context(sj, **lad_kw)
+                c.set_state(start, sj.target)
+                if mode == "learned":
+                    c.set_learned_lad(model, 0.05, args.ramp_min, args.learned_refresh)
+                sms, _ = timed_run(c, args.shocked_timing_steps, 0)
+                sh[f"{mode}_ms_per_step"] = sms / args.shocked_timing_steps
+                if mode == "learned":
+                    sh["fallback_fraction"] = c.fallback_fraction()
+                c.close()
+            sh["speedup_vs_lad"] = sh["lad_ms_per_step"] / sh["learned_ms_per_step"]
+            learned["shocked_npr1.5"] = sh
+            log(f"learned on the npr 1.5 jet: {sh['learned_ms_per_step']:.4f} ms/step against "
+                f"{sh['lad_ms_per_step']:.4f} under the reference LAD (x{sh['speedup_vs_lad']:.3f}), "
+                f"fallback {sh['fallback_fraction']:.4f}")
         if args.score_steps > 0 or args.shocked_score_steps > 0:
             uj = 1.5 * float(np.sqrt(jc.TJ_TINF))
             learned["score"] = {}
