@@ -475,6 +475,7 @@ static LadArgs lad_args(ajet_ctx* c, const double* q) {
   l.nseg = c->nseg;
   l.src_imin = c->lad_src_imin;
   l.str_imin = c->lad_str_imin;
+  l.zsub = 1;
   l.g = c->g;
   l.gam = c->p.gam;
   // ModShockCapturing.f90:66, 72, 79: C_mu Re, C_beta Re, C_kappa Re Pr
@@ -598,6 +599,9 @@ static hipError_t learned_fallback(ajet_ctx* c, const double* q, bool new_step, 
   l.zr_flags = c->zr_flags;
   l.src_mask = c->src_mask;
   l.str_mask = c->str_mask;
+  // a few flagged blocks: each walks its segment in parts (whole segments when every block is
+  // flagged on purpose, a negative threshold, so that the run is the reference's bit for bit)
+  l.zsub = c->sensor_thr < 0.0 ? 1 : lad_fallback_zsub(c->lad_seg);
   c->lad_restore = true;
   c->gen_written = c->gen;
   return launch_lad(l, s);

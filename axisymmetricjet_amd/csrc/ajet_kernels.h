@@ -101,6 +101,10 @@ struct LadArgs {
   // skipped: filter segments below seg_first, source tiles ending at or before column src_imin,
   // strain tiles ending at or before str_imin (the buffer holds zeros there)
   int seg_first, src_imin, str_imin;
+  // the z/r filter walks each segment as zsub sub-segments of zseg / zsub columns, each with its
+  // own warm-up (1: whole segments).  The learned mode's fallback takes lad_fallback_zsub: its few
+  // flagged blocks then finish in a fraction of the time, within 1.3e-16 of the whole-segment walk.
+  int zsub;
 };
 
 hipError_t upload_tables(const double sbp1[9][6], const double sbp2[9][6], const double tg[9][11], hipStream_t s);
@@ -112,6 +116,9 @@ int lad_zrfilter_slots(int device);
 // The filter segment length for a grid of nz columns, ramp[0..nz), ngroups line groups and
 // slots resident blocks (see ajet_lad.hip).
 int lad_zfilter_segment(const double* ramp, int nz, int ngroups, int slots);
+// Sub-segments per segment of the learned mode's fallback filter launch: 8, 4, 2 or 1, the most
+// whose length stays a multiple of 16 columns.
+int lad_fallback_zsub(int seg);
 void lad_skip_bounds(const double* ramp, int nz, int seg, int& seg_first, int& src_imin, int& str_imin);
 // the dilatation sensor of the learned mode: flags[group(j) * nseg + i / seg] = gen where
 // max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min (columns ifirst on, where the ramp
