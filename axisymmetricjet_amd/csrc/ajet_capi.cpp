@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -67,6 +68,7 @@ struct ajet_ctx {
   float *mlpB = nullptr, *mlpShift = nullptr, *mlpScale = nullptr;
   float out_scale[2] = {1.0f, 1.0f};
   double sensor_thr = 0.0, ramp_min = 0.0;
+  int fallback_parts = 0;                  // parts a flagged filter segment is walked in (0: eight)
   int refresh = 1;                         // network every refresh steps
   long long learned_steps = 0;             // steps since the network last ran (0: run it next)
   uint32_t gen = 0;                        // stamp of the sensor's flags (one per step)
@@ -601,7 +603,7 @@ static hipError_t learned_fallback(ajet_ctx* c, const double* q, bool new_step, 
   l.str_mask = c->str_mask;
   // a few flagged blocks: each walks its segment in parts (whole segments when every block is
   // flagged on purpose, a negative threshold, so that the run is the reference's bit for bit)
-  l.zsub = c->sensor_thr < 0.0 ? 1 : lad_fallback_zsub(c->lad_seg);
+  l.zsub = c->sensor_thr < 0.0 ? 1 : lad_fallback_zsub(c->lad_seg, c->fallback_parts);
   c->lad_restore = true;
   c->gen_written = c->gen;
   return launch_lad(l, s);
@@ -895,6 +897,16 @@ int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_thres
     return fail(c, AJET_EINVAL, "the learned coefficient needs shock capturing with non-zero C_mu, C_beta, C_kappa");
   if (!(ramp_min >= 0.0) || sensor_threshold != sensor_threshold) return fail(c, AJET_EINVAL, "bad threshold");
   if (refresh < 1) return fail(c, AJET_EINVAL, "refresh must be at least 1");
+  // AJET_FALLBACK_PARTS: the most parts a flagged filter segment is walked in (unset: eight;
+  // 1: whole segments, the order of the reference-LAD launches)
+  int parts = 0;
+  if (const char* env = std::getenv("AJET_FALLBACK_PARTS")) {
+    char* end = nullptr;
+    const long v = std::strtol(env, &end, 10);
+    if (end == env || *end != '\0' || (v != 1 && v != 2 && v != 4 && v != 8))
+      return fail(c, AJET_EINVAL, "AJET_FALLBACK_PARTS must be 1, 2, 4 or 8");
+    parts = (int)v;
+  }
   HIPCHK(c, hipSetDevice(c->device));
   const Geometry& g = c->g;
   if (!c->mlpW) {
@@ -937,6 +949,7 @@ int ajet_set_learned_lad(ajet_ctx* c, const ajet_mlp* model, double sensor_thres
   c->out_scale[1] = model->out_scale[1];
   c->sensor_thr = sensor_threshold;
   c->ramp_min = ramp_min;
+  c->fallback_parts = parts;
   // the sensor starts at the first column whose ramp exceeds ramp_min (the ramp rises with z)
   c->sensor_i0 = g.nz;
   for (int i = 0; i < g.nz; ++i)

@@ -116,9 +116,9 @@ int lad_zrfilter_slots(int device);
 // The filter segment length for a grid of nz columns, ramp[0..nz), ngroups line groups and
 // slots resident blocks (see ajet_lad.hip).
 int lad_zfilter_segment(const double* ramp, int nz, int ngroups, int slots);
-// Sub-segments per segment of the learned mode's fallback filter launch: 8, 4, 2 or 1, the most
-// whose length stays a multiple of 16 columns.
-int lad_fallback_zsub(int seg);
+// Sub-segments per segment of the learned mode's fallback filter launch: parts (a power of two;
+// 0: eight), or the most below it whose length stays a multiple of 16 columns.
+int lad_fallback_zsub(int seg, int parts);
 void lad_skip_bounds(const double* ramp, int nz, int seg, int& seg_first, int& src_imin, int& str_imin);
 // the dilatation sensor of the learned mode: flags[group(j) * nseg + i / seg] = gen where
 // max(0, -dil) max(dz, dr) / sqrt(T) > thr and ramp > ramp_min (columns ifirst on, where the ramp

@@ -971,8 +971,8 @@ int lad_zfilter_segment(const double* ramp, int nz, int ngroups, int slots) {
 #ifndef AJET_FALLBACK_ZSUB
 #define AJET_FALLBACK_ZSUB 8
 #endif
-int lad_fallback_zsub(int seg) {
-  for (int n = AJET_FALLBACK_ZSUB; n > 1; n /= 2)
+int lad_fallback_zsub(int seg, int parts) {
+  for (int n = parts > 0 ? parts : AJET_FALLBACK_ZSUB; n > 1; n /= 2)
     if (seg % (ZCH * n) == 0) return n;
   return 1;
 }

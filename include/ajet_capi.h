@@ -208,7 +208,10 @@ typedef struct ajet_mlp {
  * step that flags none.  The sensor of a state is taken after the step that produced it (same
  * state, same flags) and publishes whether it flagged a block to pinned host memory, so the host
  * knows before the next step's first launch: each step then waits once (polling) for the
- * previous one.  sensor_threshold < 0 reproduces the reference's LAD exactly. */
+ * previous one.  sensor_threshold < 0 reproduces the reference's LAD exactly.  A flagged block's
+ * z filter walks its segment in parts, each started 48 columns early from the unfiltered values
+ * (within 1.3e-16 of the whole-segment walk): at most AJET_FALLBACK_PARTS (environment, read
+ * here: 1, 2, 4 or 8; unset: 8) parts, 1 for sensor_threshold < 0. */
 int ajet_set_learned_lad(ajet_ctx* ctx, const ajet_mlp* model, double sensor_threshold, double ramp_min,
                          int32_t refresh);
 

@@ -262,6 +262,38 @@ def test_unflagged_blocks_return_to_the_network_values():
     assert found, "no threshold made the sensor let go of a block"
 
 
+def test_fallback_filter_parts_match_whole_segments(monkeypatch):
+    """A flagged block's z filter walks its segment in up to eight parts, each started 48 columns
+    early from the unfiltered values; the recursion damps that start by 0.467^48 < 1.3e-16.
+    Runs whose flagged blocks take whole segments (one part) and eight parts agree to rounding:
+    1e-13 of each variable's maximum after six steps with blocks flagged at every step."""
+    case, q = compressed_jet()
+    m = zero_model()
+    runs = {}
+    for parts in ("1", "8"):
+        monkeypatch.setenv("AJET_FALLBACK_PARTS", parts)
+        c = capi.Context(case, **LAD)
+        c.set_state(q, case.target)
+        c.set_learned_lad(m, sensor_threshold=0.05, ramp_min=1e-8, refresh=3)
+        fracs = []
+        for _ in range(6):
+            c.run(1)
+            fracs.append(c.fallback_fraction())
+        print(f"parts {parts}: flagged fractions {['%.3f' % f for f in fracs]}", flush=True)
+        assert min(fracs) > 0.0
+        runs[parts] = c.state()
+    scale = np.abs(runs["1"]).max(axis=(0, 1))
+    diff = np.abs(runs["8"] - runs["1"]).max(axis=(0, 1))
+    assert np.all(diff[scale == 0.0] == 0.0)  # V_theta stays zero in both
+    err = (diff[scale > 0.0] / scale[scale > 0.0]).max()
+    print(f"eight parts against one: max relative {err:.2e}", flush=True)
+    assert err <= 1e-13
+    monkeypatch.setenv("AJET_FALLBACK_PARTS", "3")
+    c = capi.Context(case, **LAD)
+    with pytest.raises(capi.AjetError):
+        c.set_learned_lad(m, sensor_threshold=0.05, ramp_min=1e-8)
+
+
 def test_hooks_leave_a_learned_run_unchanged():
     """ajet_rhs and ajet_lad_coeffs take the reference's coefficients of their input; on a
     learned context the next step restores the network's values, so the run goes on exactly as
