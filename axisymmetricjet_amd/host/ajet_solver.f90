@@ -25,7 +25,7 @@ MODULE ModuleVariables
   CHARACTER(LEN=256) :: forcing_file = 'Vz_Control_Vr_Feedback.dat'
   ! LearnedLAD = 1: the learned LAD coefficient (ajet_set_learned_lad) with the network of
   ! mlp_file, a raw little-endian copy of the ajet_mlp struct
-  INTEGER :: LearnedLAD = 0, mlp_refresh = 8
+  INTEGER :: LearnedLAD = 0, mlp_refresh = 12
   CHARACTER(LEN=256) :: mlp_file = 'learned_lad.bin'
   REAL(REAL64) :: sensor_threshold = 0.1D0, ramp_min = 1.0D-8
   INTEGER :: noutput = 200

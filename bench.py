@@ -149,7 +149,7 @@ def main(argv=None):
                     help="the ensemble's runs diverge after several hundred steps under the reference's "
                          "numerics, so 400 is the default rather than 2000")
     ap.add_argument("--learned-steps", type=int, default=1000, help="steps of the learned-LAD mode timed (0: skip)")
-    ap.add_argument("--learned-refresh", type=int, default=8, help="the network runs every this many steps")
+    ap.add_argument("--learned-refresh", type=int, default=12, help="the network runs every this many steps")
     ap.add_argument("--sensor-threshold", type=float, default=0.1)
     ap.add_argument("--ramp-min", type=float, default=1e-8)
     ap.add_argument("--score-steps", type=int, default=4000,
