@@ -513,7 +513,7 @@ def main(argv=None):
 
     cpu = None
     if not args.no_cpu:
-        # every core this job may use: the GPU box gives a one-GPU job a share of its host cores
+        # every core this job may use: a machine shared by several GPU jobs gives each a share of its cores
         # (OMP_NUM_THREADS), whatever os.cpu_count() shows there
         host_cpus = os.cpu_count() or 1
         threads = min(host_cpus, int(os.environ.get("OMP_NUM_THREADS") or host_cpus))
